@@ -1,0 +1,292 @@
+#!/usr/bin/env python3
+"""Benchmark: RK4 steps/s of the 256^3 decaying isotropic Smagorinsky LES, FP64.
+
+Workload (one step = one iteration of the reference driver loop,
+src/SM_solver.py:39-68, without I/O): N = 256, Smagorinsky Cs = 0.17,
+Delta = dx = 2 pi/N, nu = 1e-4, RK4, dt from the CFL rule (0.5) evaluated on
+the device from max(u) of the state, and U / curl written back as the
+reference's step leaves them.  Initial field: seeded solenoidal random field
+with E(k) ~ k^4 exp(-2 (k/4)^2), u_rms = 1 (synthetic).
+
+Prints one JSON line (rank 0, one GPU).  Timing: W warm-up steps, then K
+steps between two torch.cuda events on the stream the kernels run on.  The
+roofline object reports the dominant kernel's algorithmic bytes per launch
+over its mean launch time (hipEvents around each launch during K extra
+profiled steps).  The CPU baseline times the oracle (numpy/scipy FP64
+restatement of the reference, same FFT count and temporaries) for one RK4
+step of the same 256^3 LES on the host cores.
+"""
+
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (ROOT, os.path.join(ROOT, "spectral-les_amd")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+HBM_PEAK_GBS = 8000.0
+KINDS = ["x_inverse", "y_inverse", "z_fused", "y_forward", "x_forward_update", "cfl_reduce", "physical_out"]
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def field_bytes(N, precision):
+    s = 8 if precision == "fp64" else 4
+    R = N ** 3 * s
+    C = N * N * (N // 2 + 1) * 2 * s
+    return R, C
+
+
+def kernel_traffic_fields(les):
+    """(complex fields read, complex fields written) per launch of each pass;
+    the compulsory traffic of the pass (every operand read once, every
+    result written once)."""
+    if les:
+        return {
+            "x_inverse": (3, 6),
+            "y_inverse": (6, 9),
+            "z_fused": (9, 8),
+            "y_forward": (8, 6),
+            # stages 1, 2 (the mean over the 4 stages is used below)
+            "x_forward_update": (6 + 9, 6),
+        }
+    return {
+        "x_inverse": (3, 6),
+        "y_inverse": (6, 6),
+        "z_fused": (6, 3),
+        "y_forward": (3, 3),
+        "x_forward_update": (3 + 9, 6),
+    }
+
+
+def xfwd_mean_fields(les):
+    """Mean (read + write) complex fields of the x-forward/update pass over the
+    4 RK4 stages: stage 0 reads W + U_hat and writes U_hat0, U_hat1, U_hat;
+    stages 1-2 read W + U_hat, U_hat0, U_hat1 and write U_hat1, U_hat;
+    stage 3 reads W + U_hat, U_hat1 and writes U_hat1, U_hat."""
+    w = 6 if les else 3
+    per_stage = [w + 3 + 9, w + 9 + 6, w + 9 + 6, w + 6 + 6]
+    return sum(per_stage) / 4.0
+
+
+def survey_step_bytes(N, precision, les):
+    """Algorithmic bytes per RK4 step, 4 [n_T (R + C) + 15 C] (minimal
+    transform set, each transform one read + one write; RK state 15 C)."""
+    R, C = field_bytes(N, precision)
+    nT = 21 if les else 9
+    return 4 * (nT * (R + C) + 15 * C)
+
+
+def make_domain(N, precision, seed=1):
+    from sles.initial import isotropic_spectrum, set_spectral_state
+    from sles.initialize_domain import InitializeDomain
+
+    import torch
+
+    d = InitializeDomain(
+        N=N, nu=1e-4, method="RK4", LES_ON=1, LES_model="Smagorinsky", Cs=0.17, CFL=0.5, precision=precision
+    )
+    Uh = isotropic_spectrum(N, k0=4.0, seed=seed, urms=1.0, dtype=d.cdtype)
+    set_spectral_state(d, Uh)
+    del Uh
+    torch.cuda.synchronize()
+    return d
+
+
+def run_gpu(N, precision, steps, warmup, profile_steps=3):
+    import torch
+
+    from sles import _lib, fused
+    from sles.SM_solver import compute_dt, find_max_velocity
+
+    d = make_domain(N, precision)
+    # first dt with CFL 0.3 (src/SM_solver.py:21-23), later ones on the device with CFL 0.5
+    umax = find_max_velocity(d.U)
+    d.dt = compute_dt(d.U, umax, d.nu, dx=d.dx, CFL=0.3)
+    fused.rk4_step(d, dt=d.dt)
+    for _ in range(warmup):
+        fused.rk4_step(d, dt_device=True, cfl=0.5)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(steps):
+        fused.rk4_step(d, dt_device=True, cfl=0.5)
+    e1.record(stream)
+    e1.synchronize()
+    ms = e0.elapsed_time(e1)
+    scal = d._scalars.cpu().numpy()
+    if not (scal[1] >= 1e-6):
+        raise RuntimeError(f"CFL dt guard fired (dt = {scal[1]}), benchmark invalid")
+    # per-kernel timing over a few more steps
+    ms_k = (ctypes.c_double * 8)()
+    cnt_k = (ctypes.c_long * 8)()
+    _lib.call("sles_profile_enable", 1)
+    for _ in range(profile_steps):
+        fused.rk4_step(d, dt_device=True, cfl=0.5)
+    torch.cuda.synchronize()
+    _lib.call("sles_profile_read", ms_k, cnt_k, 7)
+    _lib.call("sles_profile_enable", 0)
+    kern = {KINDS[i]: (ms_k[i], cnt_k[i]) for i in range(7)}
+    out = dict(ms_total=ms, steps=steps, dt_last=float(scal[1]), umax_last=float(scal[0]), kernels=kern)
+    del d
+    torch.cuda.empty_cache()
+    return out
+
+
+def roofline_of(res, N, precision, les=True):
+    R, C = field_bytes(N, precision)
+    kern = res["kernels"]
+    fields = kernel_traffic_fields(les)
+    best = max(fields, key=lambda k: kern[k][0])
+    tot_ms, cnt = kern[best]
+    avg_s = tot_ms / cnt / 1e3
+    if best == "x_forward_update":
+        nbytes = xfwd_mean_fields(les) * C
+    elif best == "z_fused":
+        nbytes = sum(fields[best]) * C
+    else:
+        nbytes = sum(fields[best]) * C
+    achieved = nbytes / avg_s / 1e9
+    per_kernel = {}
+    for k, (t, c) in kern.items():
+        if c == 0:
+            continue
+        entry = {"ms_per_launch": round(t / c, 4), "launches": c}
+        if k in fields:
+            b = xfwd_mean_fields(les) * C if k == "x_forward_update" else sum(fields[k]) * C
+            entry["algorithmic_GBps"] = round(b / (t / c / 1e3) / 1e9, 1)
+        per_kernel[k] = entry
+    return {
+        "bound": "hbm",
+        "kernel": best,
+        "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": None,
+        "algorithmic_bytes_per_launch": int(nbytes),
+        "avg_launch_ms": round(avg_s * 1e3, 4),
+    }, per_kernel
+
+
+def cpu_baseline(N, budget_s=60.0):
+    """One RK4 step of the oracle (FP64, scipy.fft with all host cores) on the
+    same 256^3 Smagorinsky workload."""
+    import numpy as np
+
+    from oracle import sles_oracle as O
+
+    cores = min(O.cpu_count(), 16)
+    d = O.Domain(N=N, nu=1e-4, method="RK4", LES_ON=1, LES_model="Smagorinsky", Cs=0.17, fft_workers=cores)
+    rng = np.random.default_rng(0)
+    u = rng.standard_normal((3, N, N, N))
+    O.initial_velocity(d, *u)
+    d.U_hat1[:] = d.U_hat
+    d.dt = 1e-3
+    t0 = time.perf_counter()
+    O.rk4_step(d)
+    t = time.perf_counter() - t0
+    return {
+        "value": round(1.0 / t, 5),
+        "unit": "RK4 steps/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": f"1 RK4 step of the {N}^3 Smagorinsky LES (108 FFTs + numpy temporaries), "
+        f"oracle FP64 with scipy.fft workers={cores}, pointwise numpy single-threaded; {t:.1f} s",
+        "cpu": platform.processor() or platform.machine(),
+    }
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-form comparison run")
+    a = ap.parse_args(argv)
+    if a.gpus != 1:
+        print(
+            f"bench.py runs on one GPU only: --gpus {a.gpus} is not supported "
+            "(this build has no multi-GPU path; see DESIGN.md for how it would shard)",
+            file=sys.stderr,
+        )
+        return 2
+    import torch
+
+    if not torch.cuda.is_available():
+        print("bench.py needs a GPU", file=sys.stderr)
+        return 2
+    N = a.n
+    log(f"[bench] N={N} FP64 Smagorinsky RK4: warmup {a.warmup}, timed {a.steps}")
+    r64 = run_gpu(N, "fp64", a.steps, a.warmup)
+    ms_step = r64["ms_total"] / a.steps
+    value = 1e3 / ms_step
+    roof, per_kernel = roofline_of(r64, N, "fp64")
+    step_bytes = survey_step_bytes(N, "fp64", True)
+    log(f"[bench] fp64 {value:.2f} steps/s ({ms_step:.3f} ms/step); dominant {roof['kernel']} "
+        f"{roof['achieved']} GB/s")
+    extra = {
+        "grid_points_x_steps_per_s": round(value * N ** 3, 1),
+        "step_roofline_frac": round(step_bytes * value / (HBM_PEAK_GBS * 1e9), 4),
+        "step_algorithmic_GB": round(step_bytes / 1e9, 3),
+        "kernels_fp64": per_kernel,
+        "dt_last": r64["dt_last"],
+    }
+    if not a.no_fp32:
+        log("[bench] FP32 form")
+        r32 = run_gpu(N, "fp32", a.steps, a.warmup)
+        v32 = 1e3 * a.steps / r32["ms_total"]
+        extra["fp32_steps_per_s"] = round(v32, 3)
+        extra["fp32_speedup_vs_fp64"] = round(v32 / value, 3)
+        extra["kernels_fp32"] = roofline_of(r32, N, "fp32")[1]
+        log(f"[bench] fp32 {v32:.2f} steps/s")
+    cpu = None
+    if not a.no_cpu:
+        log("[bench] CPU baseline (oracle, one RK4 step)")
+        cpu = cpu_baseline(N)
+        log(f"[bench] cpu {cpu['value']} steps/s on {cpu['cores']} cores")
+    line = {
+        "metric": "RK4 steps/s at 256^3 (FP64 vs FP32 @ rel-L2<=1e-4); % HBM peak",
+        "value": round(value, 3),
+        "unit": "RK4 steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded k^4 exp(-2(k/4)^2) solenoidal field, u_rms=1)",
+        "config": {
+            "workload": f"decaying isotropic LES {N}^3, Smagorinsky Cs=0.17, nu=1e-4, RK4, CFL=0.5 (device dt), "
+            "U/curl written each step",
+            "grid": N,
+            "les_model": "Smagorinsky",
+            "scheme": "RK4",
+            "parallelism": "single GPU",
+        },
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "extra": extra,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,209 @@
+/* libsles: MI355X (gfx950) kernels for the RHS and time-step hot path of the
+ * pseudo-spectral Navier-Stokes / LES solver a-k-sangwan/spectral-LES.
+ *
+ * C ABI.  All array arguments are device pointers to caller-owned memory in
+ * the reference's layout (one rank, P = 1):
+ *   real field     (N, N, N)        float64 / float32, z fastest
+ *   spectral field (N, N, N/2+1)    complex128 / complex64 interleaved, [kx][ky][kz]
+ * A "field array" argument (void* const f[k]) lists k such fields.  Every call
+ * is asynchronous on the given hipStream_t (passed as void*), returns 0 on
+ * success and -1 on error (message from sles_last_error(), per thread), and
+ * does no host synchronisation.  Only sles_plan_create / sles_plan_destroy
+ * allocate or free device memory.  A plan must not be used from two streams
+ * at once.
+ *
+ * Reference citations are paths inside a-k-sangwan/spectral-LES.
+ */
+#ifndef SLES_H
+#define SLES_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SLES_ABI_VERSION 1
+
+enum { SLES_FP64 = 0, SLES_FP32 = 1 };
+enum { SLES_LES_NONE = 0, SLES_LES_SMAGORINSKY = 1, SLES_LES_VREMAN = 2 };
+
+typedef struct sles_plan_s* sles_plan;
+
+const char* sles_last_error(void);
+int sles_version(void);
+
+/* Plan for an N^3 periodic grid in one precision.  Replaces the transform
+ * object built by PFFT(comm, [N]*3, axes=[0,1,2], dtype=float, grid=(-1,))
+ * at src/initialize_domain.py:59-65.  Supported N: 8 12 16 24 32 48 64 96
+ * 128 192 256 384 512. */
+int sles_plan_create(sles_plan* plan, int n, int precision);
+int sles_plan_destroy(sles_plan plan);
+int sles_plan_info(sles_plan plan, int* n, int* nh, int* precision);
+
+/* fft.forward(u, out, normalize=False): batch real fields -> spectral,
+ * unscaled, times `scale` (pass 1/N^3 for normalize=True).
+ * numpy.fft.rfftn(u, axes=(0,1,2)) semantics.  Calls at
+ * src/math_formula.py:10-12,77-125, src/initial.py:13-15. */
+int sles_fft_r2c(sles_plan plan, const void* in, void* out, int batch, double scale, void* stream);
+
+/* fft.backward(U, out, normalize=True): batch spectral fields -> real, times
+ * `scale` (1/N^3 for normalize=True, 1 otherwise).  numpy.fft.irfftn
+ * semantics: x, then y, then z c2r (imaginary parts of the kz = 0 and N/2
+ * bins ignored).  `work` holds batch spectral fields; `in` is left unchanged
+ * unless `work` == `in`, which is allowed and overwrites it.  Calls at
+ * src/math_formula.py:18-24,33-61,
+ * src/RK4_method.py:32-34,41-43,97. */
+int sles_fft_c2r(sles_plan plan, const void* in, void* out, void* work, int batch, double scale,
+                 void* stream);
+
+/* ---- pointwise pieces of src/math_formula.py, smagorinsky.py, vreman.py ---- */
+
+/* c = a x b on real fields (the products inside U_cross_omega,
+ * src/math_formula.py:10-12). */
+int sles_cross_product(sles_plan plan, void* const a[3], void* const b[3], void* const c[3],
+                       void* stream);
+
+/* w = i K x u_hat on spectral fields (the operand of del_cross_U,
+ * src/math_formula.py:22-24), wavenumbers of src/initialize_domain.py:76-107. */
+int sles_spectral_curl(sles_plan plan, void* const u[3], void* const w[3], void* stream);
+
+/* g[3*i + j] = i K_i u_hat_j (operands of velocity_gradient,
+ * src/math_formula.py:33-61; a_{i+1,j+1} = d u_j / d x_i). */
+int sles_spectral_gradient(sles_plan plan, void* const u[3], void* const g[9], void* stream);
+
+/* Smagorinsky nu_R = coeff * sqrt(2 S:S), coeff = (Cs dx)^2, from the nine
+ * gradient fields a[3*i+j] (src/smagorinsky.py:24-39). */
+int sles_smagorinsky_nu(sles_plan plan, void* const a[9], void* nu, double coeff, void* stream);
+
+/* Vreman nu_R = sqrt(B_beta) * [aa > 1e-4 and sqrt(B_beta) > aa/1e7] *
+ * sqrt(1/aa) * coeff, coeff = 2.5 Cs^2 dx^2; also writes aa = sum a_ij^2 to
+ * `aa` (src/vreman.py:24-123).  NaN for B_beta < 0 as in the reference. */
+int sles_vreman_nu(sles_plan plan, void* const a[9], void* nu, void* aa, double coeff, void* stream);
+
+/* tau = (a11, (a12+a21)/2, (a13+a31)/2, a22, (a23+a32)/2, a33) * nu_R: the six
+ * operands transformed in residue_stress (src/math_formula.py:73-113). */
+int sles_stress_products(sles_plan plan, void* const a[9], const void* nu, void* const tau[6],
+                         void* stream);
+
+/* T_i = 2i dealias sum_j K_j tau_hat_ij for tau_hat = F(tau) in the order of
+ * sles_stress_products (src/math_formula.py:73-113). */
+int sles_stress_divergence(sles_plan plan, void* const tau_hat[6], void* const t[3], void* stream);
+
+/* The spectral tail of computeRHS (src/RK4_method.py:48-70): du *= dealias;
+ * P_hat = sum_i (du + T)_i K_i / K^2; du -= P_hat K; du += forcing * u_hat
+ * (k != 0); f1 = du (if f1 != NULL); du -= nu K^2 u_hat (if nu > 0);
+ * du += T (if t != NULL).  p_hat may be NULL. */
+int sles_rhs_finish(sles_plan plan, void* const du[3], void* const t[3], void* const u_hat[3],
+                    void* p_hat, void* const f1[3], double nu, double forcing, void* stream);
+
+/* out = K^2 * in for nfields spectral fields (operand of dissipation_rate,
+ * src/RK4_method.py:97). */
+int sles_mul_k2(sles_plan plan, const void* in, void* out, int nfields, void* stream);
+
+/* x *= s for nfields spectral fields (contiguous). */
+int sles_scale(sles_plan plan, void* x, int nfields, double s, void* stream);
+
+/* ---- stage updates (arrays of 3 spectral fields, contiguous) ----
+ * kind 0 (RK4, src/RK4_method.py:24-26):
+ *    if (c0 != 0) u_hat = u_hat0 + c0 * du ;   u_hat1 += c1 * du
+ *    (c0 = b[rk] dt, or 0 for the last stage; c1 = a[rk] dt)
+ * kind 1 (RK3 as written, src/RK3_method.py:29): u_hat = u_hat0 + c0 * du
+ * kind 2 (src/RK3_method.py:31): u_hat = u_hat0 - c0 * du + (2 c0) * du
+ * kind 3 (src/RK3_method.py:33): u_hat = u_hat0 + c0 * (du/6 + 2 du/3 + du/6) */
+int sles_stage_update(sles_plan plan, int kind, void* u_hat, const void* u_hat0, void* u_hat1,
+                      const void* du, double c0, double c1, void* stream);
+
+/* ETD2 updates on 3 spectral fields, linear operator L = -nu K^2, h = dt.
+ * mode 0: the reference formula of src/ETD2_method.py:16-29, literally
+ *   u = e^{hL} u + (F1 (1 + h nu K^2) + F0 (1 - h nu K^2 - e^{hL})) / (h (nu K^2)^2)
+ *   (denominator 1 at K^2 = 0).
+ * mode 1: Cox-Matthews ETD2 (product form)
+ *   u = e^{hL} u + h (phi1 + phi2)(hL) F1 - h phi2(hL) F0,
+ *   phi functions in FP64 with Taylor series near 0. */
+int sles_etd2_update(sles_plan plan, int mode, void* u_hat, const void* f1, const void* f0, double dt,
+                     double nu, void* stream);
+
+/* ---- FP64 reductions (results written to device memory) ----
+ * `work` must hold sles_reduce_work_bytes() bytes. */
+size_t sles_reduce_work_bytes(void);
+/* out[0] = sum over nfields real fields of x^2 (U_square numerator,
+ * src/RK4_method.py:77-84). */
+int sles_sum_squares(sles_plan plan, void* const f[], int nfields, double* out, void* work,
+                     void* stream);
+/* out[0] = sum_f sum_x a_f b_f (dissipation_rate sum, src/RK4_method.py:94-98). */
+int sles_dot(sles_plan plan, void* const a[], void* const b[], int nfields, double* out, void* work,
+             void* stream);
+/* out[0] = signed max over nfields real fields (numpy.max of the list U,
+ * src/SM_solver.py:245-255). */
+int sles_max(sles_plan plan, void* const f[], int nfields, double* out, void* work, void* stream);
+/* Binned spectrum of calculate_Ek (src/SM_solver.py:154-192): for bins
+ * b = 0..nbins-1 with K^2 in [(5b)^2, (5b+5)^2):
+ *   ek_sum[b] = sum Ek, ksum[b] = sum sqrt(K^2), count[b] = number of modes,
+ * Ek = sum_i |u_hat_i|^2, doubled for kz >= 1.  u_hat = 3 contiguous fields. */
+int sles_spectrum(sles_plan plan, const void* u_hat, int nbins, double* ek_sum, double* ksum,
+                  double* count, void* stream);
+
+/* ---- fused time step (src/RK4_method.py:20-71, src/RK3_method.py:20-41,
+ * src/ETD2_method.py:7-37 with src/smagorinsky.py / src/vreman.py) ----
+ * One call advances u_hat by one step of `scheme` with every RHS evaluation
+ * done in five transform passes (x, y inverse; a z kernel that does the c2r,
+ * all physical-space work and the r2c; y, x forward with dealiasing,
+ * projection, forcing, viscous term and the stage update fused in).
+ *
+ *   SLES_SCHEME_RK4   classical RK4 as in RK4_method.solveForNextTimestep;
+ *                     needs u_hat1 == u_hat on entry (SM_solver.py:33) and
+ *                     leaves u_hat0, u_hat1 as the reference does.  If aux is
+ *                     not NULL it receives F(u_hat) at the start state (the
+ *                     nonlinear term below), the F_{n-1} of a following ETD2.
+ *   SLES_SCHEME_RK3   the reference RK3 as written (first order).
+ *   SLES_SCHEME_ETD2  Cox-Matthews ETD2: aux holds F_{n-1} on entry and F_n
+ *                     on exit; F includes the SGS stress divergence.
+ *   SLES_SCHEME_RHS   aux = computeRHS(u_hat) (full right-hand side).
+ *   SLES_SCHEME_NONLINEAR aux = F(u_hat) = computeRHS without -nu K^2 u_hat.
+ *
+ * dt_device = 1: dt = min(cfl dx / max(u), 0.1 dx^2 / nu) from the signed
+ * max of u at the start state (find_max_velocity/compute_dt,
+ * src/SM_solver.py:245-277), computed on the device into scalars[0] (max u)
+ * and scalars[1] (dt); when that dt is NaN or below 1e-6 the step is not
+ * taken (u_hat unchanged) -- the host checks scalars[1].
+ * u_out / curl_out (3 real fields each, may be NULL) receive U and curl of
+ * the new state, as the reference leaves them.  `work` must hold
+ * sles_step_work_bytes(plan, les_model) bytes. */
+enum {
+  SLES_SCHEME_RK4 = 0,
+  SLES_SCHEME_RK3 = 1,
+  SLES_SCHEME_ETD2 = 2,
+  SLES_SCHEME_RHS = 3,
+  SLES_SCHEME_NONLINEAR = 4
+};
+
+typedef struct {
+  double dt;        /* step size when dt_device == 0 */
+  double nu;        /* kinematic viscosity */
+  double forcing;   /* linear forcing coefficient f (f * u_hat for k != 0) */
+  double sgs_coeff; /* (Cs dx)^2 Smagorinsky, 2.5 Cs^2 dx^2 Vreman */
+  double cfl;       /* CFL number for dt_device */
+  double dx;        /* 2 pi / N */
+  int les_model;    /* SLES_LES_NONE / SMAGORINSKY / VREMAN */
+  int dt_device;    /* 0: use dt; 1: CFL dt on the device */
+} sles_step_params;
+
+size_t sles_step_work_bytes(sles_plan plan, int les_model);
+
+/* Kernel timing of sles_step for measurement: while enabled, every launch of
+ * sles_step is bracketed by hipEvents on its stream.  sles_profile_read
+ * synchronises on them, adds the elapsed milliseconds and launch counts per
+ * kernel kind (0 x-inverse, 1 y-inverse, 2 z, 3 y-forward, 4 x-forward with
+ * the stage update, 5 CFL reduction, 6 U/curl output) into ms[]/count[]
+ * (nkinds entries) and clears the record. */
+int sles_profile_enable(int on);
+int sles_profile_read(double* ms, long* count, int nkinds);
+int sles_step(sles_plan plan, int scheme, const sles_step_params* params, void* u_hat, void* u_hat0, void* u_hat1,
+              void* aux, void* u_out, void* curl_out, double* scalars, void* work, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SLES_H */

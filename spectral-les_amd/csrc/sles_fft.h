@@ -1,0 +1,280 @@
+// Stockham FFT building blocks for gfx950 (CDNA4, wave64).
+//
+// A "line" is one 1D transform of length N held by NT = N/P threads; thread t
+// of a line holds the P points t + m*NT (m = 0..P-1) in registers.  That
+// distribution is the same on input and on output of every transform, so
+// pointwise work between an inverse and a forward transform needs no data
+// exchange, and global loads/stores along a contiguous axis are coalesced
+// (consecutive t -> consecutive addresses).
+//
+// Stage s of radix R (Ns = product of the earlier radices) runs N/R
+// butterflies; butterfly j = t + q*NT (q = 0..P/R-1) of thread t reads the
+// slots q + r*P/R, multiplies by W^(r*k*N/(Ns*R)) with k = j mod Ns, runs a
+// radix-R DFT and sends output r to position (j-k)*R + k + r*Ns.  Between
+// stages the outputs go through LDS; after the last stage every output is
+// already in the slot it started from.  (Stockham autosort, e.g. Govindaraju
+// et al., "High performance discrete Fourier transforms on graphics
+// processors", SC'08.)
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace sles {
+
+template <typename T>
+struct alignas(2 * sizeof(T)) cpx {
+  T x, y;
+};
+
+template <typename T>
+__device__ __forceinline__ cpx<T> operator+(cpx<T> a, cpx<T> b) { return {a.x + b.x, a.y + b.y}; }
+template <typename T>
+__device__ __forceinline__ cpx<T> operator-(cpx<T> a, cpx<T> b) { return {a.x - b.x, a.y - b.y}; }
+template <typename T>
+__device__ __forceinline__ cpx<T> operator*(T s, cpx<T> a) { return {s * a.x, s * a.y}; }
+template <typename T>
+__device__ __forceinline__ cpx<T> cmul(cpx<T> a, cpx<T> b) {
+  return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+template <typename T>
+__device__ __forceinline__ cpx<T> conj(cpx<T> a) { return {a.x, -a.y}; }
+// i * a
+template <typename T>
+__device__ __forceinline__ cpx<T> mul_i(cpx<T> a) { return {-a.y, a.x}; }
+// -i * a
+template <typename T>
+__device__ __forceinline__ cpx<T> mul_mi(cpx<T> a) { return {a.y, -a.x}; }
+// (i * s) * a for real s: the spectral derivative factor i*K.
+template <typename T>
+__device__ __forceinline__ cpx<T> mul_is(T s, cpx<T> a) { return {-s * a.y, s * a.x}; }
+
+// ---------------------------------------------------------------------------
+// Line plans: radices per stage and points per thread P (every radix divides P).
+// ---------------------------------------------------------------------------
+template <int N_, int P_, int... Rs>
+struct Plan {
+  static constexpr int N = N_;
+  static constexpr int P = P_;
+  static constexpr int NT = N_ / P_;
+  template <template <int...> class F>
+  using apply = F<Rs...>;
+};
+
+// Default plan per length: radix-8 where possible (fewest LDS exchanges).
+template <int N>
+struct LinePlan;
+#define SLES_PLAN(NN, PP, ...) \
+  template <>                  \
+  struct LinePlan<NN> : Plan<NN, PP, __VA_ARGS__> {};
+SLES_PLAN(8, 8, 8)
+SLES_PLAN(12, 12, 4, 3)
+SLES_PLAN(16, 4, 4, 4)
+SLES_PLAN(24, 12, 2, 3, 4)
+SLES_PLAN(32, 8, 8, 4)
+SLES_PLAN(48, 12, 4, 4, 3)
+SLES_PLAN(64, 8, 8, 8)
+SLES_PLAN(96, 12, 3, 4, 4, 2)
+SLES_PLAN(128, 8, 8, 4, 4)
+SLES_PLAN(192, 12, 4, 4, 4, 3)
+SLES_PLAN(256, 8, 8, 8, 4)
+SLES_PLAN(384, 12, 4, 4, 4, 3, 2)
+SLES_PLAN(512, 8, 8, 8, 8)
+#undef SLES_PLAN
+
+// Radix-4 plans (4 points per thread: half the registers of the radix-8
+// plans, twice the threads per line, more LDS exchanges).
+template <int N>
+struct LinePlan4 : LinePlan<N> {};
+#define SLES_PLAN4(NN, ...) \
+  template <>               \
+  struct LinePlan4<NN> : Plan<NN, 4, __VA_ARGS__> {};
+SLES_PLAN4(32, 4, 4, 2)
+SLES_PLAN4(64, 4, 4, 4)
+SLES_PLAN4(128, 4, 4, 4, 2)
+SLES_PLAN4(256, 4, 4, 4, 4)
+SLES_PLAN4(512, 4, 4, 4, 4, 2)
+#undef SLES_PLAN4
+
+// ---------------------------------------------------------------------------
+// Radix-R DFT in registers.  DIR = -1 forward (e^{-2 pi i nk/R}), +1 inverse.
+// ---------------------------------------------------------------------------
+template <typename T, int DIR>
+__device__ __forceinline__ cpx<T> rot_q(cpx<T> a) {  // multiply by e^{DIR i pi/2}
+  return DIR < 0 ? mul_mi(a) : mul_i(a);
+}
+
+template <typename T, int R, int DIR>
+struct Butterfly;
+
+template <typename T, int DIR>
+struct Butterfly<T, 2, DIR> {
+  __device__ __forceinline__ static void run(cpx<T>* a) {
+    cpx<T> t = a[0];
+    a[0] = t + a[1];
+    a[1] = t - a[1];
+  }
+};
+
+template <typename T, int DIR>
+struct Butterfly<T, 3, DIR> {
+  __device__ __forceinline__ static void run(cpx<T>* a) {
+    const T s = T(0.86602540378443864676372317075293618);  // sin(2 pi/3)
+    cpx<T> sum = a[1] + a[2];
+    cpx<T> dif = a[1] - a[2];
+    cpx<T> m = a[0] - T(0.5) * sum;
+    // forward: -i s (a1 - a2); inverse: +i s (a1 - a2)
+    cpx<T> u = DIR < 0 ? cpx<T>{s * dif.y, -s * dif.x} : cpx<T>{-s * dif.y, s * dif.x};
+    a[0] = a[0] + sum;
+    a[1] = m + u;
+    a[2] = m - u;
+  }
+};
+
+template <typename T, int DIR>
+struct Butterfly<T, 4, DIR> {
+  __device__ __forceinline__ static void run(cpx<T>* a) {
+    cpx<T> t0 = a[0] + a[2];
+    cpx<T> t1 = a[0] - a[2];
+    cpx<T> t2 = a[1] + a[3];
+    cpx<T> t3 = rot_q<T, DIR>(a[1] - a[3]);
+    a[0] = t0 + t2;
+    a[2] = t0 - t2;
+    a[1] = t1 + t3;
+    a[3] = t1 - t3;
+  }
+};
+
+template <typename T, int DIR>
+struct Butterfly<T, 8, DIR> {
+  __device__ __forceinline__ static void run(cpx<T>* a) {
+    const T h = T(0.70710678118654752440084436210484904);  // 1/sqrt(2)
+    cpx<T> e[4] = {a[0], a[2], a[4], a[6]};
+    cpx<T> o[4] = {a[1], a[3], a[5], a[7]};
+    Butterfly<T, 4, DIR>::run(e);
+    Butterfly<T, 4, DIR>::run(o);
+    // W8^k = e^{DIR i pi k/4}
+    cpx<T> o1 = DIR < 0 ? cpx<T>{h * (o[1].x + o[1].y), h * (o[1].y - o[1].x)}
+                        : cpx<T>{h * (o[1].x - o[1].y), h * (o[1].y + o[1].x)};
+    cpx<T> o2 = rot_q<T, DIR>(o[2]);
+    cpx<T> o3 = DIR < 0 ? cpx<T>{h * (o[3].y - o[3].x), -h * (o[3].x + o[3].y)}
+                        : cpx<T>{-h * (o[3].x + o[3].y), h * (o[3].x - o[3].y)};
+    a[0] = e[0] + o[0];
+    a[4] = e[0] - o[0];
+    a[1] = e[1] + o1;
+    a[5] = e[1] - o1;
+    a[2] = e[2] + o2;
+    a[6] = e[2] - o2;
+    a[3] = e[3] + o3;
+    a[7] = e[3] - o3;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// LDS index maps used for the exchange between stages.
+// ---------------------------------------------------------------------------
+// Lines stored contiguously (contiguous-axis passes): position p of the line.
+struct LdsLinear {
+  int base;
+  __device__ __forceinline__ int operator()(int p) const { return base + p; }
+};
+// Columns interleaved (strided passes): position p of column c, TK columns.
+template <int TK>
+struct LdsColumn {
+  int c;
+  __device__ __forceinline__ int operator()(int p) const { return p * TK + c; }
+};
+
+// ---------------------------------------------------------------------------
+// One stage: twiddle + butterflies on the P register slots of thread t.
+// tw[e] = exp(-2 pi i e / N) (forward table; the inverse conjugates it).
+// ---------------------------------------------------------------------------
+template <typename T, int N, int P, int DIR, int R, int Ns>
+__device__ __forceinline__ void line_stage(cpx<T> (&v)[P], int t, const cpx<T>* __restrict__ tw) {
+  constexpr int NT = N / P;
+  constexpr int Q = P / R;
+  static_assert(P % R == 0, "radix must divide points per thread");
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    cpx<T> a[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) a[r] = v[q + r * Q];
+    if constexpr (Ns > 1) {
+      const int j = t + q * NT;
+      const int k = j % Ns;
+      constexpr int step = N / (Ns * R);
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        cpx<T> w = tw[r * k * step];
+        if constexpr (DIR > 0) w.y = -w.y;
+        a[r] = cmul(a[r], w);
+      }
+    }
+    Butterfly<T, R, DIR>::run(a);
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[q + r * Q] = a[r];
+  }
+}
+
+template <typename T, int N, int P, int R, int Ns, class Map>
+__device__ __forceinline__ void line_exchange(cpx<T> (&v)[P], cpx<T>* lds, const Map& map, int t) {
+  constexpr int NT = N / P;
+  constexpr int Q = P / R;
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int j = t + q * NT;
+    const int k = j % Ns;
+    const int base = (j - k) * R + k;
+#pragma unroll
+    for (int r = 0; r < R; ++r) lds[map(base + r * Ns)] = v[q + r * Q];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < P; ++m) v[m] = lds[map(t + m * NT)];
+  __syncthreads();
+}
+
+template <typename T, int N, int P, int DIR, int Ns, int... Rs>
+struct LineStages;
+
+template <typename T, int N, int P, int DIR, int Ns, int R>
+struct LineStages<T, N, P, DIR, Ns, R> {
+  template <class Map>
+  __device__ __forceinline__ static void run(cpx<T> (&v)[P], cpx<T>* lds, const Map& map, int t,
+                                             const cpx<T>* __restrict__ tw) {
+    line_stage<T, N, P, DIR, R, Ns>(v, t, tw);
+  }
+};
+
+template <typename T, int N, int P, int DIR, int Ns, int R, int R2, int... Rest>
+struct LineStages<T, N, P, DIR, Ns, R, R2, Rest...> {
+  template <class Map>
+  __device__ __forceinline__ static void run(cpx<T> (&v)[P], cpx<T>* lds, const Map& map, int t,
+                                             const cpx<T>* __restrict__ tw) {
+    line_stage<T, N, P, DIR, R, Ns>(v, t, tw);
+    line_exchange<T, N, P, R, Ns>(v, lds, map, t);
+    LineStages<T, N, P, DIR, Ns * R, R2, Rest...>::run(v, lds, map, t, tw);
+  }
+};
+
+template <typename T, class PL, int DIR>
+struct LineFFTP {
+  static constexpr int N = PL::N;
+  static constexpr int P = PL::P;
+  static constexpr int NT = PL::NT;
+  template <int... Rs>
+  using stages = LineStages<T, N, P, DIR, 1, Rs...>;
+  using S = typename PL::template apply<stages>;
+  // Transform the line held by thread t (0 <= t < NT).  Every thread of the
+  // block must call this (it contains block barriers when the plan has more
+  // than one stage).
+  template <class Map>
+  __device__ __forceinline__ static void run(cpx<T> (&v)[P], cpx<T>* lds, const Map& map, int t,
+                                             const cpx<T>* __restrict__ tw) {
+    S::run(v, lds, map, t, tw);
+  }
+};
+
+template <typename T, int N, int DIR>
+struct LineFFT : LineFFTP<T, LinePlan<N>, DIR> {};
+
+}  // namespace sles

@@ -1,0 +1,325 @@
+// Generic batched 3D real<->complex transforms of the reference's layout:
+// physical (N, N, N) real, z fastest; spectral (N, N, N/2+1) complex [kx][ky][kz].
+//
+//   forward  = numpy.fft.rfftn(u, axes=(0,1,2))     : z r2c, then y, then x
+//   backward = numpy.fft.irfftn(U, s=(N,)*3, ...)    : x, then y, then z c2r
+//
+// (the reference's fft.forward / fft.backward, initialize_domain.py:59-65).
+// Each pass reads and writes one full array; the x/y passes run along a
+// strided axis with a tile of TK consecutive columns per block so every
+// global access is a run of TK contiguous complex values.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "sles.h"
+#include "sles_internal.h"
+
+namespace sles {
+
+static thread_local std::string g_err;
+
+void set_error(const std::string& msg) { g_err = msg; }
+int fail(const std::string& msg) {
+  g_err = msg;
+  return -1;
+}
+int check_launch(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(std::string(what) + ": " + hipGetErrorString(e));
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Contiguous-axis passes (two real rows per complex line).
+// ---------------------------------------------------------------------------
+template <typename T, int N>
+__global__ void __launch_bounds__(ZShape<N>::THREADS)
+    k_z_r2c(const T* __restrict__ in, cpx<T>* __restrict__ out, long nrows, T scale,
+            const cpx<T>* __restrict__ tw) {
+  using S = ZShape<N>;
+  constexpr int P = S::P, NT = S::NT, NH = S::NH;
+  __shared__ cpx<T> lds[S::LPB * N];
+  const int line = threadIdx.x / NT, t = threadIdx.x % NT;
+  const long ra = 2 * ((long)blockIdx.x * S::LPB + line), rb = ra + 1;
+  const bool va = ra < nrows, vb = rb < nrows;
+  cpx<T> v[P];
+#pragma unroll
+  for (int m = 0; m < P; ++m) {
+    const int z = t + m * NT;
+    v[m].x = va ? in[ra * N + z] : T(0);
+    v[m].y = vb ? in[rb * N + z] : T(0);
+  }
+  LineFFT<T, N, -1>::run(v, lds, LdsLinear{line * N}, t, tw);
+  // Split Z = FFT(a + i b):  A_k = (Z_k + conj Z_{N-k})/2,  B_k = (Z_k - conj Z_{N-k})/(2i)
+#pragma unroll
+  for (int m = 0; m < P; ++m) lds[line * N + t + m * NT] = v[m];
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m <= P / 2; ++m) {
+    const int k = t + m * NT;
+    if (k <= N / 2) {
+      const cpx<T> zk = v[m];
+      const cpx<T> zn = conj(lds[line * N + ((N - k) % N)]);
+      const cpx<T> a = T(0.5) * (zk + zn);
+      const cpx<T> d = zk - zn;
+      const cpx<T> b = {T(0.5) * d.y, T(-0.5) * d.x};
+      if (va) out[ra * NH + k] = scale * a;
+      if (vb) out[rb * NH + k] = scale * b;
+    }
+  }
+}
+
+template <typename T, int N>
+__global__ void __launch_bounds__(ZShape<N>::THREADS)
+    k_z_c2r(const cpx<T>* __restrict__ in, T* __restrict__ out, long nrows, T scale,
+            const cpx<T>* __restrict__ tw) {
+  using S = ZShape<N>;
+  constexpr int P = S::P, NT = S::NT, NH = S::NH;
+  __shared__ cpx<T> lds[S::LPB * N];
+  const int line = threadIdx.x / NT, t = threadIdx.x % NT;
+  const long ra = 2 * ((long)blockIdx.x * S::LPB + line), rb = ra + 1;
+  const bool va = ra < nrows, vb = rb < nrows;
+  cpx<T> v[P];
+#pragma unroll
+  for (int m = 0; m < P; ++m) {
+    const int k = t + m * NT;
+    cpx<T> a = {T(0), T(0)}, b = {T(0), T(0)};
+    if (k <= N / 2) {
+      if (va) a = in[ra * NH + k];
+      if (vb) b = in[rb * NH + k];
+      if (k == 0 || k == N / 2) {  // c2r ignores the imaginary part of these bins
+        a.y = T(0);
+        b.y = T(0);
+      }
+    } else {
+      if (va) a = conj(in[ra * NH + (N - k)]);
+      if (vb) b = conj(in[rb * NH + (N - k)]);
+    }
+    v[m] = {a.x - b.y, a.y + b.x};  // a + i b
+  }
+  LineFFT<T, N, +1>::run(v, lds, LdsLinear{line * N}, t, tw);
+#pragma unroll
+  for (int m = 0; m < P; ++m) {
+    const int z = t + m * NT;
+    if (va) out[ra * N + z] = scale * v[m].x;
+    if (vb) out[rb * N + z] = scale * v[m].y;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Strided complex pass: batch of fields, each viewed as (outer, N, M) with the
+// transform along the middle axis.  Column g in [0, outer*M) -> (g / M, g % M).
+// ---------------------------------------------------------------------------
+template <typename T, int N, int DIR>
+__global__ void __launch_bounds__((StridedShape<T, N>::THREADS))
+    k_strided(const cpx<T>* in, cpx<T>* out, long outer, long M, long fstride, T scale,
+              const cpx<T>* __restrict__ tw) {
+  using S = StridedShape<T, N>;
+  constexpr int P = S::P, NT = S::NT, TK = S::TK;
+  __shared__ cpx<T> lds[N * TK];
+  const int c = threadIdx.x % TK, t = threadIdx.x / TK;
+  const long g = (long)blockIdx.x * TK + c;
+  const bool valid = g < outer * M;
+  const long o = valid ? g / M : 0, col = valid ? g % M : 0;
+  const long base = (long)blockIdx.y * fstride + o * (long)N * M + col;
+  cpx<T> v[P];
+#pragma unroll
+  for (int m = 0; m < P; ++m) v[m] = valid ? in[base + (long)(t + m * NT) * M] : cpx<T>{T(0), T(0)};
+  LineFFT<T, N, DIR>::run(v, lds, LdsColumn<TK>{c}, t, tw);
+  if (valid) {
+#pragma unroll
+    for (int m = 0; m < P; ++m) out[base + (long)(t + m * NT) * M] = scale * v[m];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers
+// ---------------------------------------------------------------------------
+template <typename T, int N>
+static int launch_z_r2c(const sles_plan_s* p, const void* in, void* out, long nrows, double scale,
+                        hipStream_t s) {
+  using S = ZShape<N>;
+  const long pairs = (nrows + 1) / 2;
+  const long blocks = (pairs + S::LPB - 1) / S::LPB;
+  hipLaunchKernelGGL((k_z_r2c<T, N>), dim3((unsigned)blocks), dim3(S::THREADS), 0, s, (const T*)in,
+                     (cpx<T>*)out, nrows, (T)scale, (const cpx<T>*)p->tw);
+  return check_launch("z_r2c");
+}
+
+template <typename T, int N>
+static int launch_z_c2r(const sles_plan_s* p, const void* in, void* out, long nrows, double scale,
+                        hipStream_t s) {
+  using S = ZShape<N>;
+  const long pairs = (nrows + 1) / 2;
+  const long blocks = (pairs + S::LPB - 1) / S::LPB;
+  hipLaunchKernelGGL((k_z_c2r<T, N>), dim3((unsigned)blocks), dim3(S::THREADS), 0, s,
+                     (const cpx<T>*)in, (T*)out, nrows, (T)scale, (const cpx<T>*)p->tw);
+  return check_launch("z_c2r");
+}
+
+template <typename T, int N>
+static int launch_strided(const sles_plan_s* p, const void* in, void* out, long outer, long M,
+                          long fstride, int nfields, int dir, double scale, hipStream_t s) {
+  using S = StridedShape<T, N>;
+  const long blocks = (outer * M + S::TK - 1) / S::TK;
+  if (dir < 0)
+    hipLaunchKernelGGL((k_strided<T, N, -1>), dim3((unsigned)blocks, nfields), dim3(S::THREADS), 0, s,
+                       (const cpx<T>*)in, (cpx<T>*)out, outer, M, fstride, (T)scale,
+                       (const cpx<T>*)p->tw);
+  else
+    hipLaunchKernelGGL((k_strided<T, N, +1>), dim3((unsigned)blocks, nfields), dim3(S::THREADS), 0, s,
+                       (const cpx<T>*)in, (cpx<T>*)out, outer, M, fstride, (T)scale,
+                       (const cpx<T>*)p->tw);
+  return check_launch("strided");
+}
+
+template <typename T>
+static int fft_r2c_t(const sles_plan_s* p, const void* in, void* out, int batch, double scale,
+                     hipStream_t s) {
+  const int n = p->n;
+  const long C = ncplx(p);
+  const long rows = (long)batch * n * n;
+  int rc = 0;
+#define SLES_CASE(NN)                                                                     \
+  case NN:                                                                                \
+    rc = launch_z_r2c<T, NN>(p, in, out, rows, 1.0, s);                                   \
+    if (!rc) rc = launch_strided<T, NN>(p, out, out, (long)n, (long)p->nh, C, batch, -1, 1.0, s); \
+    if (!rc) rc = launch_strided<T, NN>(p, out, out, 1, (long)n * p->nh, C, batch, -1, scale, s); \
+    break;
+  switch (n) {
+    SLES_FOR_EACH_N(SLES_CASE)
+    default:
+      return fail("unsupported grid size");
+  }
+#undef SLES_CASE
+  return rc;
+}
+
+template <typename T>
+static int fft_c2r_t(const sles_plan_s* p, const void* in, void* out, void* work, int batch,
+                     double scale, hipStream_t s) {
+  const int n = p->n;
+  const long C = ncplx(p);
+  const long rows = (long)batch * n * n;
+  int rc = 0;
+#define SLES_CASE(NN)                                                                      \
+  case NN:                                                                                 \
+    rc = launch_strided<T, NN>(p, in, work, 1, (long)n * p->nh, C, batch, +1, 1.0, s);      \
+    if (!rc) rc = launch_strided<T, NN>(p, work, work, (long)n, (long)p->nh, C, batch, +1, 1.0, s); \
+    if (!rc) rc = launch_z_c2r<T, NN>(p, work, out, rows, scale, s);                       \
+    break;
+  switch (n) {
+    SLES_FOR_EACH_N(SLES_CASE)
+    default:
+      return fail("unsupported grid size");
+  }
+#undef SLES_CASE
+  return rc;
+}
+
+// Internal entry points shared with the fused time-step code.
+int z_c2r_rows(const sles_plan_s* p, const void* in, void* out, long nrows, double scale, hipStream_t s) {
+  int rc = 0;
+#define SLES_CASE(NN)                                                                         \
+  case NN:                                                                                    \
+    rc = p->precision == SLES_FP64 ? launch_z_c2r<double, NN>(p, in, out, nrows, scale, s)    \
+                                   : launch_z_c2r<float, NN>(p, in, out, nrows, scale, s);    \
+    break;
+  switch (p->n) {
+    SLES_FOR_EACH_N(SLES_CASE)
+    default:
+      return fail("unsupported grid size");
+  }
+#undef SLES_CASE
+  return rc;
+}
+
+}  // namespace sles
+
+using namespace sles;
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char* sles_last_error(void) { return g_err.c_str(); }
+
+int sles_version(void) { return SLES_ABI_VERSION; }
+
+int sles_plan_create(sles_plan* out, int n, int precision) {
+  if (!out) return fail("sles_plan_create: null output pointer");
+  *out = nullptr;
+  if (!sles_supported_n(n)) return fail("sles_plan_create: unsupported grid size " + std::to_string(n));
+  if (precision != SLES_FP64 && precision != SLES_FP32)
+    return fail("sles_plan_create: precision must be SLES_FP64 or SLES_FP32");
+  sles_plan_s* p = new sles_plan_s();
+  p->n = n;
+  p->nh = n / 2 + 1;
+  p->precision = precision;
+  (void)hipGetDevice(&p->device);
+  // Twiddles exp(-2 pi i e/n) evaluated in long double and rounded once.
+  const size_t esz = precision == SLES_FP64 ? 16 : 8;
+  std::vector<unsigned char> host(esz * n);
+  for (int e = 0; e < n; ++e) {
+    const long double ang = -2.0L * 3.14159265358979323846264338327950288L * (long double)e / (long double)n;
+    const long double c = cosl(ang), sn = sinl(ang);
+    if (precision == SLES_FP64) {
+      double v[2] = {(double)c, (double)sn};
+      std::memcpy(&host[esz * e], v, esz);
+    } else {
+      float v[2] = {(float)c, (float)sn};
+      std::memcpy(&host[esz * e], v, esz);
+    }
+  }
+  if (hipMalloc(&p->tw, esz * n) != hipSuccess) {
+    delete p;
+    return fail("sles_plan_create: hipMalloc failed for twiddle table");
+  }
+  if (hipMemcpy(p->tw, host.data(), esz * n, hipMemcpyHostToDevice) != hipSuccess) {
+    (void)hipFree(p->tw);
+    delete p;
+    return fail("sles_plan_create: twiddle upload failed");
+  }
+  *out = p;
+  return 0;
+}
+
+int sles_plan_destroy(sles_plan p) {
+  if (!p) return 0;
+  if (p->tw) (void)hipFree(p->tw);
+  delete p;
+  return 0;
+}
+
+int sles_plan_info(sles_plan p, int* n, int* nh, int* precision) {
+  if (!p) return fail("sles_plan_info: null plan");
+  if (n) *n = p->n;
+  if (nh) *nh = p->nh;
+  if (precision) *precision = p->precision;
+  return 0;
+}
+
+int sles_fft_r2c(sles_plan p, const void* in, void* out, int batch, double scale, void* stream) {
+  if (!p || !in || !out) return fail("sles_fft_r2c: null argument");
+  if (batch < 1) return fail("sles_fft_r2c: batch must be >= 1");
+  hipStream_t s = (hipStream_t)stream;
+  return p->precision == SLES_FP64 ? fft_r2c_t<double>(p, in, out, batch, scale, s)
+                                   : fft_r2c_t<float>(p, in, out, batch, scale, s);
+}
+
+int sles_fft_c2r(sles_plan p, const void* in, void* out, void* work, int batch, double scale,
+                 void* stream) {
+  if (!p || !in || !out || !work) return fail("sles_fft_c2r: null argument");
+  if (batch < 1) return fail("sles_fft_c2r: batch must be >= 1");
+  hipStream_t s = (hipStream_t)stream;
+  return p->precision == SLES_FP64 ? fft_c2r_t<double>(p, in, out, work, batch, scale, s)
+                                   : fft_c2r_t<float>(p, in, out, work, batch, scale, s);
+}
+
+}  // extern "C"

@@ -1,0 +1,86 @@
+// Internal declarations shared by the libsles translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "sles_fft.h"
+
+struct sles_plan_s {
+  int n;          // grid points per axis (cubic grid)
+  int nh;         // n/2 + 1 spectral bins on the last axis
+  int precision;  // 0 = FP64, 1 = FP32
+  int device;
+  void* tw;       // device table tw[e] = exp(-2 pi i e/n), e in [0, n)
+};
+
+namespace sles {
+
+void set_error(const std::string& msg);
+int fail(const std::string& msg);
+int check_launch(const char* what);
+
+inline long nreal(const sles_plan_s* p) { return (long)p->n * p->n * p->n; }
+inline long ncplx(const sles_plan_s* p) { return (long)p->n * p->n * p->nh; }
+
+// Wavenumbers of the reference layout (initialize_domain.py:76-79,99-107):
+// axes 0 and 1 use fftfreq(N, 1/N) (index N/2 -> -N/2); axis 2 holds 0..N/2
+// with the last entry +N/2.
+__device__ __forceinline__ int kfreq(int i, int n) { return i < n / 2 ? i : i - n; }
+
+// Spectral element e of a (N, N, N/2+1) array -> (kx, ky, kz).
+struct KIndex {
+  int kx, ky, kz;
+};
+__device__ __forceinline__ KIndex kindex(long e, int n, int nh) {
+  const int iz = (int)(e % nh);
+  const long r = e / nh;
+  const int iy = (int)(r % n);
+  const int ix = (int)(r / n);
+  return {kfreq(ix, n), kfreq(iy, n), iz};
+}
+
+// dealias = prod_i |K_i| < 2/3 (N//2 + 1)   (initialize_domain.py:79,109-114)
+__device__ __forceinline__ bool dealias_keep(KIndex k, int n) {
+  const double kmax = 2.0 / 3.0 * (double)(n / 2 + 1);
+  return (double)abs(k.kx) < kmax && (double)abs(k.ky) < kmax && (double)abs(k.kz) < kmax;
+}
+
+// Shapes of the generic passes.
+template <int N>
+struct ZShape {
+  static constexpr int P = LinePlan<N>::P;
+  static constexpr int NT = N / P;
+  static constexpr int LPB = (256 / NT) > 0 ? (256 / NT) : 1;  // lines per block
+  static constexpr int THREADS = LPB * NT;
+  static constexpr int NH = N / 2 + 1;
+};
+
+// Strided passes: TK columns per block, LDS tile N*TK complex <= 64 KiB,
+// at most 1024 threads.
+template <typename T, int N, class PL = LinePlan<N>>
+struct StridedShape {
+  static constexpr int P = PL::P;
+  static constexpr int NT = N / P;
+  static constexpr int TK0 = 65536 / (N * (int)sizeof(cpx<T>));
+  static constexpr int TK1 = TK0 > 64 ? 64 : TK0;
+  static constexpr int TK = (TK1 * NT > 1024) ? 1024 / NT : TK1;
+  static constexpr int THREADS = TK * NT;
+};
+
+int z_c2r_rows(const sles_plan_s* p, const void* in, void* out, long nrows, double scale, hipStream_t s);
+
+}  // namespace sles
+
+// Sizes with a compiled line plan.
+#define SLES_FOR_EACH_N(X) \
+  X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512)
+
+inline bool sles_supported_n(int n) {
+#define SLES_CASE(NN) if (n == NN) return true;
+  SLES_FOR_EACH_N(SLES_CASE)
+#undef SLES_CASE
+  return false;
+}

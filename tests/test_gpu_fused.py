@@ -1,0 +1,148 @@
+"""GPU parity of the fused time-step kernels (sles_step through the C ABI)
+against the CPU oracle on identical seeded fields.
+
+Tolerances (rel-L2 of U_hat, FP64): 1e-12 for one RHS / one step, 1e-10
+after a trajectory; FP32 form: 1e-5 after one step."""
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from helpers import random_field, rel, twins
+from oracle import sles_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+LES = [None, "Smagorinsky", "Vreman"]
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _need_gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+
+
+def _kw(les, **extra):
+    kw = dict(LES_ON=1, LES_model=les, Cs=0.17) if les else {}
+    kw.update(extra)
+    return kw
+
+
+@pytest.mark.parametrize("les", LES)
+@pytest.mark.parametrize("N", [12, 16, 32])
+@pytest.mark.parametrize("forcing", [0.0, 0.2])
+def test_fused_rhs(les, N, forcing):
+    from sles import fused
+
+    g, o, u = twins(N, seed=N + 1, fused=True, **_kw(les, nu=0.01, forcing=forcing, method="RK4"))
+    out = torch.zeros_like(g.U_hat)
+    fused.rhs(g, out)
+    O.computeRHS(o, o.dU, 1)
+    assert rel(out, o.dU) < 1e-12
+    nl = torch.zeros_like(g.U_hat)
+    fused.nonlinear(g, nl)
+    assert rel(nl, o.dU + o.nu * o.K2 * o.U_hat) < 1e-12
+
+
+@pytest.mark.parametrize("les", LES)
+@pytest.mark.parametrize("N", [16, 24, 48])
+def test_fused_rk4_steps(les, N):
+    import sles.RK4_method as R
+
+    g, o, u = twins(N, seed=2, fused=True, **_kw(les, nu=0.005, time_stepSize=0.01, method="RK4"))
+    for _ in range(3):
+        R.solveForNextTimestep(g)
+        O.rk4_step(o)
+    assert rel(g.U_hat, o.U_hat) < 1e-11
+    assert rel(g.U_hat1, o.U_hat1) < 1e-11
+    assert rel(g.U_hat0, o.U_hat0) < 1e-11
+    assert rel(torch.stack(g.U), np.array(o.U)) < 1e-11
+    assert rel(g.curl, o.curl) < 1e-11
+
+
+@pytest.mark.parametrize("les", [None, "Vreman"])
+def test_fused_rk3(les):
+    import sles.RK3_method as R3
+
+    g, o, u = twins(16, seed=4, fused=True, **_kw(les, nu=0.005, time_stepSize=0.01, method="RK3"))
+    for _ in range(3):
+        R3.solveForNextTimestep(g)
+        O.rk3_step(o)
+    assert rel(g.U_hat, o.U_hat) < 1e-11
+
+
+@pytest.mark.parametrize("les", [None, "Smagorinsky", "Vreman"])
+def test_fused_etd2_cox_matthews(les):
+    import sles.ETD2_method as E
+
+    g, o, u = twins(16, seed=6, fused=True, **_kw(les, nu=0.01, time_stepSize=0.005, method="ETD2"))
+    for _ in range(4):
+        E.solveForNextTimestep(g)
+        O.etd2_step_cox_matthews(o)
+    assert rel(g.U_hat, o.U_hat) < 1e-11
+
+
+@pytest.mark.parametrize("les", [None, "Smagorinsky"])
+def test_fused_device_dt_matches_cfl_rule(les):
+    from sles import fused
+
+    N = 32
+    g, o, u = twins(N, seed=8, fused=True, **_kw(les, nu=0.002, method="RK4", CFL=0.5))
+    umax = O.find_max_velocity(o.U)
+    dt = O.compute_dt(o.U, umax, o.nu, dx=o.dx, CFL=0.5)
+    o.dt = dt
+    fused.rk4_step(g, dt_device=True, cfl=0.5)
+    O.rk4_step(o)
+    s = g._scalars.cpu().numpy()
+    assert math.isclose(s[0], umax, rel_tol=1e-13)
+    assert math.isclose(s[1], dt, rel_tol=1e-13)
+    assert rel(g.U_hat, o.U_hat) < 1e-11
+
+
+def test_fused_device_dt_guard_skips_step():
+    from sles import fused
+
+    g, o, u = twins(16, seed=9, fused=True, nu=0.01, method="RK4")
+    before = g.U_hat.clone()
+    # a tiny CFL number forces dt < 1e-6: the step must not be taken
+    fused.rk4_step(g, dt_device=True, cfl=1e-9)
+    assert g._scalars[1].item() < 1e-6
+    assert torch.equal(g.U_hat, before)
+
+
+@pytest.mark.parametrize("les", [None, "Smagorinsky", "Vreman"])
+def test_fused_fp32_form(les):
+    import sles.RK4_method as R
+
+    g, o, u = twins(32, seed=10, precision="fp32", fused=True,
+                    **_kw(les, nu=0.005, time_stepSize=0.01, method="RK4"))
+    R.solveForNextTimestep(g)
+    O.rk4_step(o)
+    assert rel(g.U_hat, o.U_hat) < 1e-5
+
+
+def test_taylor_green_64_rk4_trajectory():
+    """64^3 Taylor-Green, nu = 1/1600, dt = 0.01, RK4, no LES (first 50 of the
+    200 steps of the CPU-runnable case; the full run is in the slow test)."""
+    import sles.RK4_method as R
+    from sles.initial import initial_velocity
+    from sles.initialize_domain import InitializeDomain
+
+    N = 64
+    x, y, z = O.grid(N)
+    u = np.sin(x) * np.cos(y) * np.cos(z)
+    v = -np.cos(x) * np.sin(y) * np.cos(z)
+    w = np.zeros_like(u)
+    g = InitializeDomain(N=N, nu=1 / 1600, time_stepSize=0.01, method="RK4")
+    o = O.Domain(N=N, nu=1 / 1600, time_stepSize=0.01, method="RK4")
+    initial_velocity(g, u, v, w)
+    O.initial_velocity(o, u, v, w)
+    g.U_hat1.copy_(g.U_hat)
+    o.U_hat1[:] = o.U_hat
+    for _ in range(50):
+        R.solveForNextTimestep(g)
+        O.rk4_step(o)
+    assert rel(torch.stack(g.U), np.array(o.U)) < 1e-10
+    assert rel(g.U_hat, o.U_hat) < 1e-10
