@@ -41,6 +41,19 @@ int sles_plan_create(sles_plan* plan, int n, int precision);
 int sles_plan_destroy(sles_plan plan);
 int sles_plan_info(sles_plan plan, int* n, int* nh, int* precision);
 
+/* Kernel variants of the fused step, chosen per plan (by the autotuner,
+ * sles/autotune.py, which persists its choices in a JSON table).  Value 0 is
+ * the default.  No counterpart in the reference.
+ *   SLES_TUNE_STRIDED_RADIX4  x/y inverse + y forward passes use the radix-4
+ *                             line plan (4 points per thread) instead of radix-8
+ *   SLES_TUNE_XFWD_RADIX4     same for the x-forward + stage-update pass
+ *   SLES_TUNE_Z_RADIX4        same for the fused z kernel
+ *   SLES_TUNE_XCD             XCD-aware tile order in the strided passes
+ * The radix-4 plans exist for N = 32, 64, 128, 256, 512 (ignored otherwise). */
+enum { SLES_TUNE_STRIDED_RADIX4 = 0, SLES_TUNE_XFWD_RADIX4 = 1, SLES_TUNE_Z_RADIX4 = 2, SLES_TUNE_XCD = 3 };
+int sles_plan_set_tuning(sles_plan plan, int key, int value);
+int sles_plan_get_tuning(sles_plan plan, int key, int* value);
+
 /* fft.forward(u, out, normalize=False): batch real fields -> spectral,
  * unscaled, times `scale` (pass 1/N^3 for normalize=True).
  * numpy.fft.rfftn(u, axes=(0,1,2)) semantics.  Calls at
@@ -201,6 +214,27 @@ int sles_profile_enable(int on);
 int sles_profile_read(double* ms, long* count, int nkinds);
 int sles_step(sles_plan plan, int scheme, const sles_step_params* params, void* u_hat, void* u_hat0, void* u_hat1,
               void* aux, void* u_out, void* curl_out, double* scalars, void* work, void* stream);
+
+/* ---- hand-written adjoint (no counterpart in the reference) ----
+ * out (=, or += when accumulate) J(y)^T mu: the vector-Jacobian product of
+ * computeRHS (src/RK4_method.py:38-71, with the closure of les_model) at the
+ * state y, for an adjoint mu of the RHS; complex arrays are taken as real
+ * vectors (Re, Im), the convention of torch gradients.  linear = 1 includes
+ * the linear part (forcing [k != 0] - nu K^2); linear = 2 only the forcing
+ * (the nonlinear term F that ETD2 uses); linear = 0 neither.  If
+ * cs_grad is not NULL, cs_grad[0] += sum_x nu_bar d nu_R / d Cs with
+ * d sgs_coeff / d Cs = dcoeff (2 Cs dx^2 Smagorinsky, 5 Cs dx^2 Vreman).
+ * `work` must hold sles_adjoint_work_bytes(plan, les_model) bytes. */
+size_t sles_adjoint_work_bytes(sles_plan plan, int les_model);
+int sles_adjoint_rhs(sles_plan plan, const sles_step_params* params, const void* y, const void* mu, void* out,
+                     int accumulate, int linear, double dcoeff, double* cs_grad, void* work, void* stream);
+/* ETD2 adjoint combination (Cox-Matthews, L = -nu K^2, h = dt, FP64 phi):
+ *   mu = h (phi1 + phi2)(hL) lam1 - h phi2(hL) lam2 ;  lam_out = e^{hL} lam1
+ * lam2 may be NULL (taken as 0).  3 spectral fields each. */
+int sles_etd2_adjoint_combine(sles_plan plan, double dt, double nu, const void* lam1, const void* lam2, void* mu,
+                              void* lam_out, void* stream);
+/* y = a x + b y over nfields spectral fields (contiguous). */
+int sles_axpby(sles_plan plan, int nfields, double a, const void* x, double b, void* y, void* stream);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,93 @@
+// C ABI of the hand-written adjoint RHS (kernels in sles_adjoint.h) and the
+// small vector helpers the adjoint time stepping uses.
+#include "sles_adjoint.h"
+
+namespace sles {
+
+template <typename T>
+__global__ void k_axpby(long n, double a, const cpx<T>* x, double b, cpx<T>* y) {
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x)
+    y[e] = (T)a * x[e] + (T)b * y[e];
+}
+
+__device__ __forceinline__ double phi1_e(double z) {
+  if (fabs(z) < 1e-2) return 1 + z / 2 * (1 + z / 3 * (1 + z / 4 * (1 + z / 5 * (1 + z / 6))));
+  return expm1(z) / z;
+}
+__device__ __forceinline__ double phi2_e(double z) {
+  if (fabs(z) < 1e-2) return 0.5 * (1 + z / 3 * (1 + z / 4 * (1 + z / 5 * (1 + z / 6 * (1 + z / 7)))));
+  return (expm1(z) - z) / (z * z);
+}
+
+template <typename T>
+__global__ void k_etd2_adj(long nc, int n, int nh, double h, double nu, const cpx<T>* l1, const cpx<T>* l2,
+                           cpx<T>* mu, cpx<T>* lo) {
+  const long total = 3 * nc;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
+    const KIndex k = kindex(e % nc, n, nh);
+    const long k2 = (long)k.kx * k.kx + (long)k.ky * k.ky + (long)k.kz * k.kz;
+    const double z = -h * nu * (double)k2;
+    const double p1 = phi1_e(z), p2 = phi2_e(z);
+    const cpx<T> a = l1[e];
+    const cpx<T> b = l2 ? l2[e] : cpx<T>{T(0), T(0)};
+    mu[e] = (T)(h * (p1 + p2)) * a - (T)(h * p2) * b;
+    lo[e] = (T)exp(z) * a;
+  }
+}
+
+}  // namespace sles
+
+using namespace sles;
+
+extern "C" {
+
+size_t sles_adjoint_work_bytes(sles_plan p, int les_model) {
+  (void)les_model;
+  if (!p) return 0;
+  const size_t esz = p->precision == SLES_FP64 ? 16 : 8;
+  return 17 * (size_t)ncplx(p) * esz + sizeof(double) * ((size_t)p->n * p->n + 64);
+}
+
+int sles_adjoint_rhs(sles_plan p, const sles_step_params* prm, const void* y, const void* mu, void* out,
+                     int accumulate, int linear, double dcoeff, double* cs_grad, void* work, void* stream) {
+  if (!p || !prm || !y || !mu || !out || !work) return fail("sles_adjoint_rhs: null argument");
+  if (prm->les_model < 0 || prm->les_model > 2) return fail("sles_adjoint_rhs: les_model must be 0, 1 or 2");
+  if (out == mu || out == y) return fail("sles_adjoint_rhs: out must not alias y or mu");
+  if (!sles_supported_n(p->n)) return fail("sles_adjoint_rhs: unsupported grid size");
+  hipStream_t s = (hipStream_t)stream;
+  return p->precision == SLES_FP64 ? adjoint_f64(p, prm, y, mu, out, accumulate, linear, dcoeff, cs_grad, work, s)
+                                   : adjoint_f32(p, prm, y, mu, out, accumulate, linear, dcoeff, cs_grad, work, s);
+}
+
+int sles_etd2_adjoint_combine(sles_plan p, double dt, double nu, const void* lam1, const void* lam2, void* mu,
+                              void* lam_out, void* stream) {
+  if (!p || !lam1 || !mu || !lam_out) return fail("sles_etd2_adjoint_combine: null argument");
+  const long nc = ncplx(p);
+  long blocks = (3 * nc + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  if (p->precision == SLES_FP64)
+    hipLaunchKernelGGL(k_etd2_adj<double>, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, nc, p->n, p->nh,
+                       dt, nu, (const cpx<double>*)lam1, (const cpx<double>*)lam2, (cpx<double>*)mu,
+                       (cpx<double>*)lam_out);
+  else
+    hipLaunchKernelGGL(k_etd2_adj<float>, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, nc, p->n, p->nh,
+                       dt, nu, (const cpx<float>*)lam1, (const cpx<float>*)lam2, (cpx<float>*)mu,
+                       (cpx<float>*)lam_out);
+  return check_launch("sles_etd2_adjoint_combine");
+}
+
+int sles_axpby(sles_plan p, int nfields, double a, const void* x, double b, void* y, void* stream) {
+  if (!p || !x || !y || nfields < 1) return fail("sles_axpby: bad argument");
+  const long n = (long)nfields * ncplx(p);
+  long blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  if (p->precision == SLES_FP64)
+    hipLaunchKernelGGL(k_axpby<double>, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, n, a,
+                       (const cpx<double>*)x, b, (cpx<double>*)y);
+  else
+    hipLaunchKernelGGL(k_axpby<float>, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, n, a,
+                       (const cpx<float>*)x, b, (cpx<float>*)y);
+  return check_launch("sles_axpby");
+}
+
+}  // extern "C"

@@ -21,6 +21,17 @@
 
 namespace sles {
 
+// Block barrier that is also a compiler barrier for memory operations.
+// __syncthreads() alone (fence + s_barrier + fence) has let the scheduler sink
+// LDS reads of an exchange below the barrier that protects the next writes
+// (observed on gfx950 with FP32 lines), which races between waves.  The
+// "memory" clobbers keep every load and store on its side of the barrier.
+__device__ __forceinline__ void block_sync() {
+  asm volatile("" ::: "memory");
+  __syncthreads();
+  asm volatile("" ::: "memory");
+}
+
 template <typename T>
 struct alignas(2 * sizeof(T)) cpx {
   T x, y;
@@ -227,10 +238,10 @@ __device__ __forceinline__ void line_exchange(cpx<T> (&v)[P], cpx<T>* lds, const
 #pragma unroll
     for (int r = 0; r < R; ++r) lds[map(base + r * Ns)] = v[q + r * Q];
   }
-  __syncthreads();
+  block_sync();
 #pragma unroll
   for (int m = 0; m < P; ++m) v[m] = lds[map(t + m * NT)];
-  __syncthreads();
+  block_sync();
 }
 
 template <typename T, int N, int P, int DIR, int Ns, int... Rs>

@@ -47,18 +47,20 @@ __global__ void __launch_bounds__(ZShape<N>::THREADS)
   const int line = threadIdx.x / NT, t = threadIdx.x % NT;
   const long ra = 2 * ((long)blockIdx.x * S::LPB + line), rb = ra + 1;
   const bool va = ra < nrows, vb = rb < nrows;
+  const long rla = va ? ra : 0, rlb = vb ? rb : 0;  // clamped rows (in bounds)
   cpx<T> v[P];
 #pragma unroll
   for (int m = 0; m < P; ++m) {
     const int z = t + m * NT;
-    v[m].x = va ? in[ra * N + z] : T(0);
-    v[m].y = vb ? in[rb * N + z] : T(0);
+    const T xa = in[rla * N + z], xb = in[rlb * N + z];
+    v[m].x = va ? xa : T(0);
+    v[m].y = vb ? xb : T(0);
   }
   LineFFT<T, N, -1>::run(v, lds, LdsLinear{line * N}, t, tw);
   // Split Z = FFT(a + i b):  A_k = (Z_k + conj Z_{N-k})/2,  B_k = (Z_k - conj Z_{N-k})/(2i)
 #pragma unroll
   for (int m = 0; m < P; ++m) lds[line * N + t + m * NT] = v[m];
-  __syncthreads();
+  block_sync();
 #pragma unroll
   for (int m = 0; m <= P / 2; ++m) {
     const int k = t + m * NT;
@@ -84,21 +86,21 @@ __global__ void __launch_bounds__(ZShape<N>::THREADS)
   const int line = threadIdx.x / NT, t = threadIdx.x % NT;
   const long ra = 2 * ((long)blockIdx.x * S::LPB + line), rb = ra + 1;
   const bool va = ra < nrows, vb = rb < nrows;
+  const long rla = va ? ra : 0, rlb = vb ? rb : 0;  // clamped rows (in bounds)
   cpx<T> v[P];
 #pragma unroll
   for (int m = 0; m < P; ++m) {
     const int k = t + m * NT;
-    cpx<T> a = {T(0), T(0)}, b = {T(0), T(0)};
-    if (k <= N / 2) {
-      if (va) a = in[ra * NH + k];
-      if (vb) b = in[rb * NH + k];
-      if (k == 0 || k == N / 2) {  // c2r ignores the imaginary part of these bins
-        a.y = T(0);
-        b.y = T(0);
-      }
-    } else {
-      if (va) a = conj(in[ra * NH + (N - k)]);
-      if (vb) b = conj(in[rb * NH + (N - k)]);
+    const int kk = k <= N / 2 ? k : N - k;
+    cpx<T> a = in[rla * NH + kk], b = in[rlb * NH + kk];
+    if (!va) a = {T(0), T(0)};
+    if (!vb) b = {T(0), T(0)};
+    if (k > N / 2) {
+      a = conj(a);
+      b = conj(b);
+    } else if (k == 0 || k == N / 2) {  // c2r ignores the imaginary part of these bins
+      a.y = T(0);
+      b.y = T(0);
     }
     v[m] = {a.x - b.y, a.y + b.x};  // a + i b
   }
@@ -129,7 +131,7 @@ __global__ void __launch_bounds__((StridedShape<T, N>::THREADS))
   const long base = (long)blockIdx.y * fstride + o * (long)N * M + col;
   cpx<T> v[P];
 #pragma unroll
-  for (int m = 0; m < P; ++m) v[m] = valid ? in[base + (long)(t + m * NT) * M] : cpx<T>{T(0), T(0)};
+  for (int m = 0; m < P; ++m) v[m] = in[base + (long)(t + m * NT) * M];  // in bounds (clamped); unused if !valid
   LineFFT<T, N, DIR>::run(v, lds, LdsColumn<TK>{c}, t, tw);
   if (valid) {
 #pragma unroll
@@ -303,6 +305,28 @@ int sles_plan_info(sles_plan p, int* n, int* nh, int* precision) {
   if (nh) *nh = p->nh;
   if (precision) *precision = p->precision;
   return 0;
+}
+
+int sles_plan_set_tuning(sles_plan p, int key, int value) {
+  if (!p) return fail("sles_plan_set_tuning: null plan");
+  switch (key) {
+    case SLES_TUNE_STRIDED_RADIX4: p->tune_strided = value != 0; return 0;
+    case SLES_TUNE_XFWD_RADIX4: p->tune_xfwd = value != 0; return 0;
+    case SLES_TUNE_Z_RADIX4: p->tune_z = value != 0; return 0;
+    case SLES_TUNE_XCD: p->tune_xcd = value != 0; return 0;
+    default: return fail("sles_plan_set_tuning: unknown key");
+  }
+}
+
+int sles_plan_get_tuning(sles_plan p, int key, int* value) {
+  if (!p || !value) return fail("sles_plan_get_tuning: null argument");
+  switch (key) {
+    case SLES_TUNE_STRIDED_RADIX4: *value = p->tune_strided; return 0;
+    case SLES_TUNE_XFWD_RADIX4: *value = p->tune_xfwd; return 0;
+    case SLES_TUNE_Z_RADIX4: *value = p->tune_z; return 0;
+    case SLES_TUNE_XCD: *value = p->tune_xcd; return 0;
+    default: return fail("sles_plan_get_tuning: unknown key");
+  }
 }
 
 int sles_fft_r2c(sles_plan p, const void* in, void* out, int batch, double scale, void* stream) {

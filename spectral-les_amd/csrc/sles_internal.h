@@ -14,6 +14,11 @@ struct sles_plan_s {
   int precision;  // 0 = FP64, 1 = FP32
   int device;
   void* tw;       // device table tw[e] = exp(-2 pi i e/n), e in [0, n)
+  // kernel variants (sles_plan_set_tuning): 0 = radix-8 line plan, 1 = radix-4
+  int tune_strided = 0;  // x/y inverse and y forward passes
+  int tune_xfwd = 0;     // x forward + stage update
+  int tune_z = 0;        // fused z kernel
+  int tune_xcd = 0;      // XCD-aware tile order in the strided passes
 };
 
 namespace sles {
@@ -58,15 +63,18 @@ struct ZShape {
   static constexpr int NH = N / 2 + 1;
 };
 
-// Strided passes: TK columns per block, LDS tile N*TK complex <= 64 KiB,
-// at most 1024 threads.
+// Strided passes: TK columns per block, LDS tile N*TK complex <= 64 KiB.
+// Blocks of plans with >= 8 points per thread are capped at 512 threads (8
+// waves: up to 256 VGPRs per wave without spilling); radix-4 plans (4 points
+// per thread, about half the registers) may use 1024.
 template <typename T, int N, class PL = LinePlan<N>>
 struct StridedShape {
   static constexpr int P = PL::P;
   static constexpr int NT = N / P;
+  static constexpr int MAXT = P >= 8 ? 512 : 1024;
   static constexpr int TK0 = 65536 / (N * (int)sizeof(cpx<T>));
   static constexpr int TK1 = TK0 > 64 ? 64 : TK0;
-  static constexpr int TK = (TK1 * NT > 1024) ? 1024 / NT : TK1;
+  static constexpr int TK = (TK1 * NT > MAXT) ? (MAXT / NT > 0 ? MAXT / NT : 1) : TK1;
   static constexpr int THREADS = TK * NT;
 };
 

@@ -273,7 +273,7 @@ __device__ double block_reduce(double v) {
   }
   const int wid = threadIdx.x / 64, lane = threadIdx.x % 64;
   if (lane == 0) s[wid] = v;
-  __syncthreads();
+  block_sync();
   if (wid == 0) {
     v = lane < (int)(blockDim.x / 64) ? s[lane] : (OP == 0 ? 0.0 : -INFINITY);
 #pragma unroll
@@ -322,7 +322,7 @@ __global__ void __launch_bounds__(kThreads) k_spectrum(const cpx<T>* uh, long nc
                                                        double* ek, double* ks, double* cnt) {
   __shared__ double s_ek[kMaxBins], s_k[kMaxBins], s_c[kMaxBins];
   for (int i = threadIdx.x; i < nbins; i += blockDim.x) s_ek[i] = s_k[i] = s_c[i] = 0.0;
-  __syncthreads();
+  block_sync();
   for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < nc; e += (long)gridDim.x * blockDim.x) {
     const KIndex k = kindex(e, n, nh);
     const long k2 = (long)k.kx * k.kx + (long)k.ky * k.ky + (long)k.kz * k.kz;
@@ -344,7 +344,7 @@ __global__ void __launch_bounds__(kThreads) k_spectrum(const cpx<T>* uh, long nc
     atomicAdd(&s_k[bin], sqrt((double)k2));
     atomicAdd(&s_c[bin], 1.0);
   }
-  __syncthreads();
+  block_sync();
   for (int i = threadIdx.x; i < nbins; i += blockDim.x) {
     if (s_c[i] != 0.0) {
       atomicAdd(&ek[i], s_ek[i]);

@@ -1,0 +1,897 @@
+// Fused RHS + time-step kernels.
+//
+// One right-hand-side evaluation of RK4_method.computeRHS
+// (src/RK4_method.py:38-71, with Smagorinsky_model / Vreman_model,
+// src/smagorinsky.py:8-47, src/vreman.py:8-123, and math_formula.py) plus the
+// stage update runs as five transform passes over a workspace W of up to 9
+// spectral fields.  Wavenumbers come from indices; the physical fields never
+// touch HBM.
+//
+//   LES (Smagorinsky / Vreman)              no LES
+//   1 x-inverse  U_hat(3) -> W0-2 = u, W3-5 = i Kx u       W0-2 = u, W3-5 = i K x u
+//   2 y-inverse  W0-5 -> + W6-8 = i Ky u (9 fields)        6 fields in place
+//   3 z kernel   9 half-spectra -> c2r of u, du/dx, du/dy, du/dz (12 fields)
+//                -> omega, u x omega, nu_R, tau = nu_R S, max(u)
+//                -> r2c -> W0-2 = F(u x omega)_i + 2i Kz tau_iz,
+//                          W3-7 = tau_xx, tau_xy, tau_xz, tau_yy, tau_yz
+//                                                          c2r u, omega -> u x omega -> W0-2
+//   4 y-forward  W0-2 += 2i Ky tau_iy (6 fields out)       3 fields in place
+//   5 x-forward  S_i = dealias (F_x(W_i) + 2i Kx F_x(tau_xi)); projection,
+//                forcing, -nu K^2 U_hat, and the stage update of the scheme.
+//
+// Linear algebra: dU = dealias F(u x omega) + T - P_hat K + f U_hat - nu K^2 U_hat
+// with T = 2i dealias sum_j K_j F(nu_R S_ij) and P_hat = sum_i (dU + T)_i K_i/K^2
+// is the value the reference computes; the sums are regrouped (K_j commutes
+// with the transforms along the other axes).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "sles.h"
+#include "sles_internal.h"
+
+namespace sles {
+
+enum { EPI_RK4 = 0, EPI_RK3 = 1, EPI_ETD2 = 2, EPI_RHS = 3, EPI_NONLIN = 4 };
+
+struct StepArgs {
+  double dt;             // host step size (when dt_dev == nullptr)
+  const double* dt_dev;  // on-device step size, or nullptr
+  double nu, forcing;
+  double ca, cb;  // RK4: a[s], b[s]
+  int scheme, stage;
+};
+
+// ---------------------------------------------------------------------------
+// Tile geometry of the strided passes (N * (N/2+1) columns, TK per block).
+// ---------------------------------------------------------------------------
+// With xcd != 0 (and a grid that is a multiple of 8) consecutive tiles go to
+// the same XCD: hardware dispatches block b to XCD b mod 8, so tile
+// (b mod 8) * (grid/8) + b/8 keeps neighbouring columns -- which share cache
+// lines on the y axis -- inside one L2.
+__device__ __forceinline__ long tile_id(int xcd) {
+  const long b = blockIdx.x;
+  if (xcd && (gridDim.x & 7) == 0) return (b & 7) * (long)(gridDim.x >> 3) + (b >> 3);
+  return b;
+}
+
+template <int N, int TK>
+struct Cols {
+  static constexpr int NH = N / 2 + 1;
+  static constexpr long M = (long)N * NH;
+  int c;
+  long g;
+  bool valid;
+  __device__ __forceinline__ explicit Cols(int xcd) {
+    c = threadIdx.x % TK;
+    g = tile_id(xcd) * TK + c;
+    valid = g < M;
+    if (!valid) g = 0;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// 1. x-inverse
+// ---------------------------------------------------------------------------
+template <typename T, int N, int LES, class PL>
+__global__ void __launch_bounds__((StridedShape<T, N, PL>::THREADS))
+    k_xinv(const cpx<T>* __restrict__ uh, cpx<T>* __restrict__ W, long C, const cpx<T>* __restrict__ tw, int xcd) {
+  using S = StridedShape<T, N, PL>;
+  constexpr int P = S::P, NT = S::NT, TK = S::TK;
+  using FFT = LineFFTP<T, PL, +1>;
+  __shared__ cpx<T> lds[N * TK];
+  const Cols<N, TK> col(xcd);
+  const int t = threadIdx.x / TK;
+  const LdsColumn<TK> map{col.c};
+  const T ky = (T)kfreq((int)(col.g / Cols<N, TK>::NH), N);
+  const T kz = (T)(col.g % Cols<N, TK>::NH);
+  auto at = [&](int m) { return (long)(t + m * NT) * Cols<N, TK>::M + col.g; };
+  cpx<T> v[P];
+  if constexpr (LES) {
+#pragma unroll 1
+    for (int j = 0; j < 3; ++j) {
+      cpx<T> w[P];
+#pragma unroll
+      for (int m = 0; m < P; ++m) {
+        v[m] = uh[j * C + at(m)];
+        w[m] = mul_i((T)kfreq(t + m * NT, N) * v[m]);  // 1j * K0 * U_j
+      }
+      FFT::run(v, lds, map, t, tw);
+      if (col.valid)
+#pragma unroll
+        for (int m = 0; m < P; ++m) W[j * C + at(m)] = v[m];
+      FFT::run(w, lds, map, t, tw);
+      if (col.valid)
+#pragma unroll
+        for (int m = 0; m < P; ++m) W[(3 + j) * C + at(m)] = w[m];
+    }
+  } else {
+#pragma unroll 1
+    for (int j = 0; j < 3; ++j) {
+#pragma unroll
+      for (int m = 0; m < P; ++m) v[m] = uh[j * C + at(m)];
+      FFT::run(v, lds, map, t, tw);
+      if (col.valid)
+#pragma unroll
+        for (int m = 0; m < P; ++m) W[j * C + at(m)] = v[m];
+    }
+    // omega_hat = 1j * (K x U_hat)  (math_formula.py:22-24)
+#pragma unroll 1
+    for (int j = 0; j < 3; ++j) {
+#pragma unroll
+      for (int m = 0; m < P; ++m) {
+        const T kx = (T)kfreq(t + m * NT, N);
+        const long e = at(m);
+        if (j == 0) {
+          const cpx<T> u1 = uh[C + e], u2 = uh[2 * C + e];
+          v[m] = mul_i(ky * u2 - kz * u1);
+        } else if (j == 1) {
+          const cpx<T> u0 = uh[e], u2 = uh[2 * C + e];
+          v[m] = mul_i(kz * u0 - kx * u2);
+        } else {
+          const cpx<T> u0 = uh[e], u1 = uh[C + e];
+          v[m] = mul_i(kx * u1 - ky * u0);
+        }
+      }
+      FFT::run(v, lds, map, t, tw);
+      if (col.valid)
+#pragma unroll
+        for (int m = 0; m < P; ++m) W[(3 + j) * C + at(m)] = v[m];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 2. y-inverse (in place on W)
+// ---------------------------------------------------------------------------
+template <typename T, int N, int LES, class PL>
+__global__ void __launch_bounds__((StridedShape<T, N, PL>::THREADS))
+    k_yinv(cpx<T>* W, long C, const cpx<T>* __restrict__ tw, int xcd) {
+  using S = StridedShape<T, N, PL>;
+  constexpr int P = S::P, NT = S::NT, TK = S::TK, NH = N / 2 + 1;
+  using FFT = LineFFTP<T, PL, +1>;
+  __shared__ cpx<T> lds[N * TK];
+  const Cols<N, TK> col(xcd);
+  const int t = threadIdx.x / TK;
+  const LdsColumn<TK> map{col.c};
+  const long ix = col.g / NH, iz = col.g % NH;
+  auto at = [&](int m) { return ix * (long)N * NH + (long)(t + m * NT) * NH + iz; };
+  cpx<T> v[P];
+  if constexpr (LES) {
+#pragma unroll 1
+    for (int j = 0; j < 3; ++j) {
+      cpx<T> w[P];
+#pragma unroll
+      for (int m = 0; m < P; ++m) {
+        v[m] = W[j * C + at(m)];
+        w[m] = mul_i((T)kfreq(t + m * NT, N) * v[m]);  // 1j * K1 * (.)
+      }
+      FFT::run(v, lds, map, t, tw);
+      if (col.valid)
+#pragma unroll
+        for (int m = 0; m < P; ++m) W[j * C + at(m)] = v[m];
+      FFT::run(w, lds, map, t, tw);
+      if (col.valid)
+#pragma unroll
+        for (int m = 0; m < P; ++m) W[(6 + j) * C + at(m)] = w[m];
+    }
+  }
+#pragma unroll 1
+  for (int j = LES ? 3 : 0; j < 6; ++j) {
+#pragma unroll
+    for (int m = 0; m < P; ++m) v[m] = W[j * C + at(m)];
+    FFT::run(v, lds, map, t, tw);
+    if (col.valid)
+#pragma unroll
+      for (int m = 0; m < P; ++m) W[j * C + at(m)] = v[m];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 3. z kernel: one line per (x, y) row.
+// ---------------------------------------------------------------------------
+template <class PL>
+struct ZLines {
+  static constexpr int NT = PL::NT;
+  static constexpr int LPB = (256 / NT) > 0 ? (256 / NT) : 1;
+  static constexpr int THREADS = LPB * NT;
+};
+
+// Load Z_k = A_k + i B_k (Hermitian completion of two half-spectra, each
+// optionally multiplied by i*kz first; imaginary parts of the kz = 0 and N/2
+// bins dropped as c2r does).
+template <typename T, int N, int P, int NT>
+__device__ __forceinline__ void load_pair(cpx<T> (&v)[P], const cpx<T>* A, bool dzA, const cpx<T>* B, bool dzB,
+                                          int t, bool valid) {
+#pragma unroll
+  for (int m = 0; m < P; ++m) {
+    const int k = t + m * NT;
+    // A, B point at an in-bounds row even for an invalid line (never stored)
+    const int kk = k <= N / 2 ? k : N - k;
+    cpx<T> a = A[kk], b = B[kk];
+    if (dzA) a = mul_i((T)kk * a);
+    if (dzB) b = mul_i((T)kk * b);
+    if (k > N / 2) {
+      a = conj(a);
+      b = conj(b);
+    } else if (k == 0 || k == N / 2) {
+      a.y = T(0);
+      b.y = T(0);
+    }
+    v[m] = {a.x - b.y, a.y + b.x};
+  }
+}
+
+// After a forward FFT of a + i b held in v (natural layout), split into the
+// half-spectra A_k, B_k of a and b for this thread's bins k = t + m*NT <= N/2,
+// m = 0..P/2.  Uses the line's LDS row; ends with a barrier.
+template <typename T, int N, int P, int NT>
+__device__ __forceinline__ void split_pair(const cpx<T> (&v)[P], cpx<T>* row, int t, cpx<T> (&A)[P / 2 + 1],
+                                           cpx<T> (&B)[P / 2 + 1]) {
+#pragma unroll
+  for (int m = 0; m < P; ++m) row[t + m * NT] = v[m];
+  block_sync();
+#pragma unroll
+  for (int m = 0; m <= P / 2; ++m) {
+    const int k = t + m * NT;
+    const cpx<T> zk = v[m];
+    const cpx<T> zn = conj(row[(N - k) % N]);
+    const cpx<T> d = zk - zn;
+    A[m] = T(0.5) * (zk + zn);
+    B[m] = {T(0.5) * d.y, T(-0.5) * d.x};
+  }
+  block_sync();
+}
+
+template <typename T>
+__device__ __forceinline__ T nanmax_t(T a, T b) {
+  return (isnan(a) || a > b) ? a : b;
+}
+
+__device__ __forceinline__ void block_max_store(double v, double* out) {
+  __shared__ double s[16];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const double w = __shfl_xor(v, o, 64);
+    v = (isnan(v) || v > w) ? v : w;
+  }
+  const int wid = threadIdx.x / 64, lane = threadIdx.x % 64;
+  if (lane == 0) s[wid] = v;
+  block_sync();
+  if (threadIdx.x == 0) {
+    double r = s[0];
+    for (int i = 1; i < (int)(blockDim.x + 63) / 64; ++i) r = (isnan(r) || r > s[i]) ? r : s[i];
+    out[blockIdx.x] = r;
+  }
+}
+
+template <typename T, int N, class PL, int MODEL>
+__global__ void __launch_bounds__(ZLines<PL>::THREADS)
+    k_zles(cpx<T>* W, long C, T scale, T coeff, double* umax_part, const cpx<T>* __restrict__ tw) {
+  constexpr int P = PL::P, NT = PL::NT, NH = N / 2 + 1, KH = P / 2 + 1;
+  constexpr int LPB = ZLines<PL>::LPB;
+  using IFFT = LineFFTP<T, PL, +1>;
+  using FFT = LineFFTP<T, PL, -1>;
+  __shared__ cpx<T> lds[LPB * N];
+  const int line = threadIdx.x / NT, t = threadIdx.x % NT;
+  const long row = (long)blockIdx.x * LPB + line;
+  const bool valid = row < (long)N * N;
+  const long base = (valid ? row : 0) * NH;
+  cpx<T>* lrow = lds + line * N;
+  const LdsLinear map{line * N};
+  const cpx<T>* w[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) w[i] = W + i * C + base;
+
+  // ---- inverse: 6 complex transforms -> 12 physical fields (scaled 1/N^3)
+  T u0[P], u1[P], u2[P], a11[P], a12[P], a13[P], a21[P], a22[P], a23[P], a31[P], a32[P], a33[P];
+  cpx<T> v[P];
+#define SLES_INV(A, DA, B, DB, RA, RB)                             \
+  load_pair<T, N, P, NT>(v, A, DA, B, DB, t, valid);               \
+  IFFT::run(v, lds, map, t, tw);                                   \
+  _Pragma("unroll") for (int m = 0; m < P; ++m) {                  \
+    RA[m] = scale * v[m].x;                                        \
+    RB[m] = scale * v[m].y;                                        \
+  }
+  SLES_INV(w[0], false, w[1], false, u0, u1)
+  SLES_INV(w[2], false, w[0], true, u2, a31)
+  SLES_INV(w[1], true, w[2], true, a32, a33)
+  SLES_INV(w[3], false, w[4], false, a11, a12)
+  SLES_INV(w[5], false, w[6], false, a13, a21)
+  SLES_INV(w[7], false, w[8], false, a22, a23)
+#undef SLES_INV
+
+  // ---- pointwise: omega, u x omega, nu_R, tau = nu_R S_ij, max(u)
+  T q0[P], q1[P], q2[P], q3[P], q4[P], q5[P], q6[P], q7[P], q8[P];
+  T umax = -INFINITY;
+#pragma unroll
+  for (int m = 0; m < P; ++m) {
+    const T w0 = a23[m] - a32[m], w1 = a31[m] - a13[m], w2 = a12[m] - a21[m];
+    q0[m] = u1[m] * w2 - u2[m] * w1;
+    q1[m] = u2[m] * w0 - u0[m] * w2;
+    q2[m] = u0[m] * w1 - u1[m] * w0;
+    umax = nanmax_t(umax, nanmax_t(u0[m], nanmax_t(u1[m], u2[m])));
+    T nu;
+    const T s12 = a12[m] + a21[m], s13 = a13[m] + a31[m], s23 = a23[m] + a32[m];
+    if constexpr (MODEL == SLES_LES_SMAGORINSKY) {
+      const T ss = a11[m] * a11[m] + T(0.5) * (s12 * s12) + a22[m] * a22[m] + T(0.5) * (s13 * s13) +
+                   T(0.5) * (s23 * s23) + a33[m] * a33[m];
+      nu = sqrt(T(2) * ss) * coeff;
+    } else {
+      const T b11 = a11[m] * a11[m] + a21[m] * a21[m] + a31[m] * a31[m];
+      const T b22 = a12[m] * a12[m] + a22[m] * a22[m] + a32[m] * a32[m];
+      const T b33 = a13[m] * a13[m] + a23[m] * a23[m] + a33[m] * a33[m];
+      const T b12 = a11[m] * a12[m] + a21[m] * a22[m] + a31[m] * a32[m];
+      const T b13 = a11[m] * a13[m] + a21[m] * a23[m] + a31[m] * a33[m];
+      const T b23 = a12[m] * a13[m] + a22[m] * a23[m] + a32[m] * a33[m];
+      const T sb = sqrt(b11 * b22 - b12 * b12 + b11 * b33 - b13 * b13 + b22 * b33 - b23 * b23);
+      const T aa = b11 + b22 + b33;
+      const T f = (aa > T(1e-4) && sb > aa / T(1e7)) ? sqrt(T(1) / aa) : T(0);
+      nu = sb * (f * coeff);
+    }
+    q3[m] = a11[m] * nu;
+    q4[m] = T(0.5) * s12 * nu;
+    q5[m] = T(0.5) * s13 * nu;
+    q6[m] = a22[m] * nu;
+    q7[m] = T(0.5) * s23 * nu;
+    q8[m] = a33[m] * nu;
+  }
+  if (umax_part) block_max_store(valid ? (double)umax : -INFINITY, umax_part);
+
+  // ---- forward: 5 complex transforms -> 8 output half-spectra
+  cpx<T> A[KH], B[KH], t13[KH], t23[KH];
+  cpx<T>* o[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) o[i] = W + i * C + base;
+  auto kok = [&](int m) { return t + m * NT <= N / 2; };
+#define SLES_FWD(QA, QB)                                     \
+  _Pragma("unroll") for (int m = 0; m < P; ++m) v[m] = {QA[m], QB[m]}; \
+  FFT::run(v, lds, map, t, tw);                              \
+  split_pair<T, N, P, NT>(v, lrow, t, A, B);
+  SLES_FWD(q4, q5)  // tau_xy, tau_xz
+#pragma unroll
+  for (int m = 0; m < KH; ++m) {
+    t13[m] = B[m];
+    if (valid && kok(m)) {
+      o[4][t + m * NT] = A[m];
+      o[5][t + m * NT] = B[m];
+    }
+  }
+  SLES_FWD(q6, q7)  // tau_yy, tau_yz
+#pragma unroll
+  for (int m = 0; m < KH; ++m) {
+    t23[m] = B[m];
+    if (valid && kok(m)) {
+      o[6][t + m * NT] = A[m];
+      o[7][t + m * NT] = B[m];
+    }
+  }
+  SLES_FWD(q2, q8)  // N_z, tau_zz
+#pragma unroll
+  for (int m = 0; m < KH; ++m) {
+    const T kz2 = T(2) * (T)(t + m * NT);
+    if (valid && kok(m)) o[2][t + m * NT] = A[m] + mul_i(kz2 * B[m]);
+  }
+  SLES_FWD(q0, q1)  // N_x, N_y
+#pragma unroll
+  for (int m = 0; m < KH; ++m) {
+    const T kz2 = T(2) * (T)(t + m * NT);
+    if (valid && kok(m)) {
+      o[0][t + m * NT] = A[m] + mul_i(kz2 * t13[m]);
+      o[1][t + m * NT] = B[m] + mul_i(kz2 * t23[m]);
+    }
+  }
+  T zq[P];
+#pragma unroll
+  for (int m = 0; m < P; ++m) zq[m] = T(0);
+  SLES_FWD(q3, zq)  // tau_xx
+#pragma unroll
+  for (int m = 0; m < KH; ++m)
+    if (valid && kok(m)) o[3][t + m * NT] = A[m];
+#undef SLES_FWD
+}
+
+template <typename T, int N, class PL>
+__global__ void __launch_bounds__(ZLines<PL>::THREADS)
+    k_zns(cpx<T>* W, long C, T scale, double* umax_part, const cpx<T>* __restrict__ tw) {
+  constexpr int P = PL::P, NT = PL::NT, NH = N / 2 + 1, KH = P / 2 + 1;
+  constexpr int LPB = ZLines<PL>::LPB;
+  using IFFT = LineFFTP<T, PL, +1>;
+  using FFT = LineFFTP<T, PL, -1>;
+  __shared__ cpx<T> lds[LPB * N];
+  const int line = threadIdx.x / NT, t = threadIdx.x % NT;
+  const long row = (long)blockIdx.x * LPB + line;
+  const bool valid = row < (long)N * N;
+  const long base = (valid ? row : 0) * NH;
+  cpx<T>* lrow = lds + line * N;
+  const LdsLinear map{line * N};
+  cpx<T>* w[6];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) w[i] = W + i * C + base;
+  T u0[P], u1[P], u2[P], o0[P], o1[P], o2[P];
+  cpx<T> v[P];
+#define SLES_INV(A, B, RA, RB)                                     \
+  load_pair<T, N, P, NT>(v, A, false, B, false, t, valid);         \
+  IFFT::run(v, lds, map, t, tw);                                   \
+  _Pragma("unroll") for (int m = 0; m < P; ++m) {                  \
+    RA[m] = scale * v[m].x;                                        \
+    RB[m] = scale * v[m].y;                                        \
+  }
+  SLES_INV(w[0], w[1], u0, u1)
+  SLES_INV(w[2], w[3], u2, o0)
+  SLES_INV(w[4], w[5], o1, o2)
+#undef SLES_INV
+  T umax = -INFINITY;
+  T n0[P], n1[P], n2[P];
+#pragma unroll
+  for (int m = 0; m < P; ++m) {
+    n0[m] = u1[m] * o2[m] - u2[m] * o1[m];
+    n1[m] = u2[m] * o0[m] - u0[m] * o2[m];
+    n2[m] = u0[m] * o1[m] - u1[m] * o0[m];
+    umax = nanmax_t(umax, nanmax_t(u0[m], nanmax_t(u1[m], u2[m])));
+  }
+  if (umax_part) block_max_store(valid ? (double)umax : -INFINITY, umax_part);
+  cpx<T> A[KH], B[KH];
+#pragma unroll
+  for (int m = 0; m < P; ++m) v[m] = {n0[m], n1[m]};
+  FFT::run(v, lds, map, t, tw);
+  split_pair<T, N, P, NT>(v, lrow, t, A, B);
+#pragma unroll
+  for (int m = 0; m < KH; ++m)
+    if (valid && t + m * NT <= N / 2) {
+      w[0][t + m * NT] = A[m];
+      w[1][t + m * NT] = B[m];
+    }
+#pragma unroll
+  for (int m = 0; m < P; ++m) v[m] = {n2[m], T(0)};
+  FFT::run(v, lds, map, t, tw);
+  split_pair<T, N, P, NT>(v, lrow, t, A, B);
+#pragma unroll
+  for (int m = 0; m < KH; ++m)
+    if (valid && t + m * NT <= N / 2) w[2][t + m * NT] = A[m];
+}
+
+// ---------------------------------------------------------------------------
+// 4. y-forward (in place)
+// ---------------------------------------------------------------------------
+template <typename T, int N, int LES, class PL>
+__global__ void __launch_bounds__((StridedShape<T, N, PL>::THREADS))
+    k_yfwd(cpx<T>* W, long C, const cpx<T>* __restrict__ tw, int xcd) {
+  using S = StridedShape<T, N, PL>;
+  constexpr int P = S::P, NT = S::NT, TK = S::TK, NH = N / 2 + 1;
+  using FFT = LineFFTP<T, PL, -1>;
+  __shared__ cpx<T> lds[N * TK];
+  const Cols<N, TK> col(xcd);
+  const int t = threadIdx.x / TK;
+  const LdsColumn<TK> map{col.c};
+  const long ix = col.g / NH, iz = col.g % NH;
+  auto at = [&](int m) { return ix * (long)N * NH + (long)(t + m * NT) * NH + iz; };
+  cpx<T> v[P], acc[P];
+  auto load = [&](int f) {
+#pragma unroll
+    for (int m = 0; m < P; ++m) v[m] = W[f * C + at(m)];
+  };
+  auto store = [&](int f, const cpx<T>(&x)[P]) {
+    if (col.valid)
+#pragma unroll
+      for (int m = 0; m < P; ++m) W[f * C + at(m)] = x[m];
+  };
+  if constexpr (LES) {
+    // H_i = F_y(G_i) + 2i Ky F_y(tau_iy): tau_xy (W4), tau_yy (W6), tau_yz (W7)
+#pragma unroll 1
+    for (int i = 0; i < 3; ++i) {
+      load(i == 0 ? 4 : (i == 1 ? 6 : 7));
+      FFT::run(v, lds, map, t, tw);
+      if (i == 0) store(4, v);
+#pragma unroll
+      for (int m = 0; m < P; ++m) acc[m] = mul_i((T(2) * (T)kfreq(t + m * NT, N)) * v[m]);
+      load(i);
+      FFT::run(v, lds, map, t, tw);
+#pragma unroll
+      for (int m = 0; m < P; ++m) acc[m] = v[m] + acc[m];
+      store(i, acc);
+    }
+#pragma unroll 1
+    for (int r = 3; r <= 5; r += 2) {
+      load(r);
+      FFT::run(v, lds, map, t, tw);
+      store(r, v);
+    }
+  } else {
+#pragma unroll 1
+    for (int f = 0; f < 3; ++f) {
+      load(f);
+      FFT::run(v, lds, map, t, tw);
+      store(f, v);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 5. x-forward + projection + scheme epilogue
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double phi1_d(double z) {
+  if (fabs(z) < 1e-2) return 1 + z / 2 * (1 + z / 3 * (1 + z / 4 * (1 + z / 5 * (1 + z / 6))));
+  return expm1(z) / z;
+}
+__device__ __forceinline__ double phi2_d(double z) {
+  if (fabs(z) < 1e-2) return 0.5 * (1 + z / 3 * (1 + z / 4 * (1 + z / 5 * (1 + z / 6 * (1 + z / 7)))));
+  return (expm1(z) - z) / (z * z);
+}
+
+template <typename T>
+__device__ __forceinline__ void epilogue(cpx<T> (&s)[3], long e, int kx, int ky, int kz, int n, const StepArgs& a,
+                                         double dt, cpx<T>* uh, cpx<T>* uh0, cpx<T>* uh1, cpx<T>* aux, long C) {
+  const KIndex k = {kx, ky, kz};
+  const T keep = dealias_keep(k, n) ? T(1) : T(0);
+  const long k2 = (long)kx * kx + (long)ky * ky + (long)kz * kz;
+  const T den = (T)(k2 == 0 ? 1 : k2);
+  const T kk[3] = {(T)kx, (T)ky, (T)kz};
+  cpx<T> d[3];
+  cpx<T> p = {T(0), T(0)};
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    d[i] = keep * s[i];
+    p = p + (kk[i] / den) * d[i];
+  }
+  cpx<T> u[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    u[i] = uh[i * C + e];
+    d[i] = d[i] - kk[i] * p;
+    if (a.forcing > 0.0 && k2 != 0) d[i] = d[i] + (T)a.forcing * u[i];
+  }
+  if (a.scheme == EPI_NONLIN) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) aux[i * C + e] = d[i];
+    return;
+  }
+  if (a.scheme == EPI_ETD2) {
+    // Cox-Matthews ETD2, L = -nu K^2, phi functions in FP64
+    const double z = -dt * a.nu * (double)k2;
+    const double p1 = phi1_d(z), p2 = phi2_d(z);
+    const T ez = (T)exp(z), w1 = (T)(dt * (p1 + p2)), w0 = (T)(dt * p2);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      const cpx<T> f0 = aux[i * C + e];
+      uh[i * C + e] = (ez * u[i] + w1 * d[i]) - w0 * f0;
+      aux[i * C + e] = d[i];
+    }
+    return;
+  }
+  if (a.scheme == EPI_RK4 && a.stage == 0 && aux) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) aux[i * C + e] = d[i];  // F(u_0), kept for an ETD2 start
+  }
+  if (a.nu > 0.0) {
+    const T c = (T)(a.nu * (double)k2);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) d[i] = d[i] - c * u[i];
+  }
+  if (a.scheme == EPI_RHS) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) aux[i * C + e] = d[i];
+    return;
+  }
+  if (a.scheme == EPI_RK4) {
+    const T ca = (T)(a.ca * dt), cb = (T)(a.cb * dt);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      const long ei = i * C + e;
+      if (a.stage == 0) {
+        uh0[ei] = u[i];
+        uh1[ei] = u[i] + ca * d[i];
+        uh[ei] = u[i] + cb * d[i];
+      } else if (a.stage < 3) {
+        uh1[ei] = uh1[ei] + ca * d[i];
+        uh[ei] = uh0[ei] + cb * d[i];
+      } else {
+        const cpx<T> fin = uh1[ei] + ca * d[i];
+        uh1[ei] = fin;
+        uh[ei] = fin;
+      }
+    }
+    return;
+  }
+  // RK3 as written (src/RK3_method.py:27-33)
+  const T hdt = (T)(0.5 * dt), fdt = (T)dt, tdt = (T)(2.0 * dt);
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const long ei = i * C + e;
+    if (a.stage == 0) {
+      uh0[ei] = u[i];
+      uh[ei] = u[i] + hdt * d[i];
+    } else if (a.stage == 1) {
+      uh[ei] = (uh0[ei] - fdt * d[i]) + tdt * d[i];
+    } else {
+      const cpx<T> sm = ((T(1) / T(6)) * d[i] + (T(2) / T(3)) * d[i]) + (T(1) / T(6)) * d[i];
+      uh[ei] = uh0[ei] + fdt * sm;
+    }
+  }
+}
+
+template <typename T, int N, int LES, class PL>
+__global__ void __launch_bounds__((StridedShape<T, N, PL>::THREADS))
+    k_xfwd(cpx<T>* __restrict__ W, long C, cpx<T>* uh, cpx<T>* uh0, cpx<T>* uh1, cpx<T>* aux, StepArgs a,
+           const cpx<T>* __restrict__ tw, int xcd) {
+  using S = StridedShape<T, N, PL>;
+  constexpr int P = S::P, NT = S::NT, TK = S::TK, NH = N / 2 + 1;
+  using FFT = LineFFTP<T, PL, -1>;
+  __shared__ cpx<T> lds[N * TK];
+  double dt = a.dt;
+  if (a.dt_dev) {
+    dt = *a.dt_dev;
+    if (!(dt >= 1e-6)) return;  // NaN or too small: the step is not taken
+  }
+  const Cols<N, TK> col(xcd);
+  const int t = threadIdx.x / TK;
+  const LdsColumn<TK> map{col.c};
+  auto at = [&](int m) { return (long)(t + m * NT) * Cols<N, TK>::M + col.g; };
+  // a0..a2: S_i = F_x(W_i) (+ 2i Kx F_x(tau_xi) with LES).  Named arrays and
+  // a rolled field loop with uniform branches keep them in registers.
+  cpx<T> a0[P], a1[P], a2[P], v[P];
+#pragma unroll 1
+  for (int f = 0; f < (LES ? 6 : 3); ++f) {
+#pragma unroll
+    for (int m = 0; m < P; ++m) v[m] = W[f * C + at(m)];
+    FFT::run(v, lds, map, t, tw);
+    if (f < 3) {
+      if (f == 0) {
+#pragma unroll
+        for (int m = 0; m < P; ++m) a0[m] = v[m];
+      } else if (f == 1) {
+#pragma unroll
+        for (int m = 0; m < P; ++m) a1[m] = v[m];
+      } else {
+#pragma unroll
+        for (int m = 0; m < P; ++m) a2[m] = v[m];
+      }
+    } else {
+#pragma unroll
+      for (int m = 0; m < P; ++m) v[m] = mul_i((T(2) * (T)kfreq(t + m * NT, N)) * v[m]);
+      if (f == 3) {
+#pragma unroll
+        for (int m = 0; m < P; ++m) a0[m] = a0[m] + v[m];
+      } else if (f == 4) {
+#pragma unroll
+        for (int m = 0; m < P; ++m) a1[m] = a1[m] + v[m];
+      } else {
+#pragma unroll
+        for (int m = 0; m < P; ++m) a2[m] = a2[m] + v[m];
+      }
+    }
+  }
+  if (!col.valid) return;
+  const int ky = kfreq((int)(col.g / NH), N), kz = (int)(col.g % NH);
+#pragma unroll
+  for (int m = 0; m < P; ++m) {
+    cpx<T> s3[3] = {a0[m], a1[m], a2[m]};
+    epilogue<T>(s3, at(m), kfreq(t + m * NT, N), ky, kz, N, a, dt, uh, uh0, uh1, aux, C);
+  }
+}
+
+// dt = min(CFL dx / max(u), 0.1 dx^2 / nu)   (src/SM_solver.py:245-272)
+static __global__ void k_cfl(const double* part, int np, double cfl, double dx, double nu, double* scalars) {
+  double m = -INFINITY;
+  for (int i = threadIdx.x; i < np; i += blockDim.x) m = (isnan(m) || m > part[i]) ? m : part[i];
+  __shared__ double s[1024];
+  s[threadIdx.x] = m;
+  block_sync();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) {
+      const double a = s[threadIdx.x], b = s[threadIdx.x + o];
+      s[threadIdx.x] = (isnan(a) || a > b) ? a : b;
+    }
+    block_sync();
+  }
+  if (threadIdx.x == 0) {
+    const double umax = s[0];
+    const double dt1 = cfl * dx / umax;
+    const double dt2 = nu > 0 ? 0.1 * dx * dx / nu : 1e6;
+    scalars[0] = umax;
+    scalars[1] = (dt2 < dt1) ? dt2 : dt1;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Optional per-kernel timing (measurement only).
+// ---------------------------------------------------------------------------
+enum { PK_XINV = 0, PK_YINV = 1, PK_Z = 2, PK_YFWD = 3, PK_XFWD = 4, PK_CFL = 5, PK_PHYS = 6, PK_N = 7 };
+struct ProfRec {
+  int kind;
+  hipEvent_t a, b;
+};
+extern bool g_prof;
+extern std::vector<ProfRec> g_recs;
+extern std::vector<hipEvent_t> g_pool;
+
+inline hipEvent_t prof_event() {
+  if (g_pool.empty()) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+  }
+  hipEvent_t e = g_pool.back();
+  g_pool.pop_back();
+  return e;
+}
+
+struct ProfScope {
+  int kind;
+  hipStream_t s;
+  hipEvent_t a = nullptr;
+  ProfScope(int k, hipStream_t st) : kind(k), s(st) {
+    if (g_prof && (a = prof_event())) (void)hipEventRecord(a, s);
+  }
+  ~ProfScope() {
+    if (!a) return;
+    hipEvent_t b = prof_event();
+    if (!b) return;
+    (void)hipEventRecord(b, s);
+    g_recs.push_back({kind, a, b});
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Host orchestration
+// ---------------------------------------------------------------------------
+template <typename T, int N, class PS, class PX, class PZ>
+struct Fused {
+  using SS = StridedShape<T, N, PS>;  // x/y inverse and y forward
+  using SX = StridedShape<T, N, PX>;  // x forward + stage update
+  static constexpr int ZB = (int)(((long)N * N + ZLines<PZ>::LPB - 1) / ZLines<PZ>::LPB);
+
+  static int stage(const sles_plan_s* p, int les, T coeff, cpx<T>* uh, cpx<T>* W, double* part, hipStream_t s) {
+    const long C = ncplx(p);
+    const cpx<T>* tw = (const cpx<T>*)p->tw;
+    const int xcd = p->tune_xcd;
+    const dim3 gs((unsigned)((C + SS::TK - 1) / SS::TK)), bs(SS::THREADS);
+    const T scale = T(1.0 / ((double)N * N * N));
+    if (les) {
+      { ProfScope ps_(PK_XINV, s); hipLaunchKernelGGL((k_xinv<T, N, 1, PS>), gs, bs, 0, s, uh, W, C, tw, xcd); }
+      { ProfScope ps_(PK_YINV, s); hipLaunchKernelGGL((k_yinv<T, N, 1, PS>), gs, bs, 0, s, W, C, tw, xcd); }
+      ProfScope ps_(PK_Z, s);
+      if (les == SLES_LES_SMAGORINSKY)
+        hipLaunchKernelGGL((k_zles<T, N, PZ, SLES_LES_SMAGORINSKY>), dim3(ZB), dim3(ZLines<PZ>::THREADS), 0, s, W, C,
+                           scale, coeff, part, tw);
+      else
+        hipLaunchKernelGGL((k_zles<T, N, PZ, SLES_LES_VREMAN>), dim3(ZB), dim3(ZLines<PZ>::THREADS), 0, s, W, C,
+                           scale, coeff, part, tw);
+    } else {
+      { ProfScope ps_(PK_XINV, s); hipLaunchKernelGGL((k_xinv<T, N, 0, PS>), gs, bs, 0, s, uh, W, C, tw, xcd); }
+      { ProfScope ps_(PK_YINV, s); hipLaunchKernelGGL((k_yinv<T, N, 0, PS>), gs, bs, 0, s, W, C, tw, xcd); }
+      ProfScope ps_(PK_Z, s);
+      hipLaunchKernelGGL((k_zns<T, N, PZ>), dim3(ZB), dim3(ZLines<PZ>::THREADS), 0, s, W, C, scale, part, tw);
+    }
+    return check_launch("fused stage (inverse)") ? -1 : 0;
+  }
+
+  static int finish(const sles_plan_s* p, int les, cpx<T>* uh, cpx<T>* uh0, cpx<T>* uh1, cpx<T>* aux, cpx<T>* W,
+                    const StepArgs& a, hipStream_t s) {
+    const long C = ncplx(p);
+    const cpx<T>* tw = (const cpx<T>*)p->tw;
+    const int xcd = p->tune_xcd;
+    const dim3 gs((unsigned)((C + SS::TK - 1) / SS::TK)), bs(SS::THREADS);
+    const dim3 gx((unsigned)((C + SX::TK - 1) / SX::TK)), bx(SX::THREADS);
+    if (les) {
+      { ProfScope ps_(PK_YFWD, s); hipLaunchKernelGGL((k_yfwd<T, N, 1, PS>), gs, bs, 0, s, W, C, tw, xcd); }
+      { ProfScope ps_(PK_XFWD, s);
+        hipLaunchKernelGGL((k_xfwd<T, N, 1, PX>), gx, bx, 0, s, W, C, uh, uh0, uh1, aux, a, tw, xcd); }
+    } else {
+      { ProfScope ps_(PK_YFWD, s); hipLaunchKernelGGL((k_yfwd<T, N, 0, PS>), gs, bs, 0, s, W, C, tw, xcd); }
+      { ProfScope ps_(PK_XFWD, s);
+        hipLaunchKernelGGL((k_xfwd<T, N, 0, PX>), gx, bx, 0, s, W, C, uh, uh0, uh1, aux, a, tw, xcd); }
+    }
+    return check_launch("fused stage (forward)");
+  }
+
+  static int physical(const sles_plan_s* p, cpx<T>* uh, cpx<T>* W, void* u_out, void* curl_out, hipStream_t s) {
+    const long C = ncplx(p);
+    const cpx<T>* tw = (const cpx<T>*)p->tw;
+    const int xcd = p->tune_xcd;
+    const dim3 gs((unsigned)((C + SS::TK - 1) / SS::TK)), bs(SS::THREADS);
+    ProfScope ps_(PK_PHYS, s);
+    hipLaunchKernelGGL((k_xinv<T, N, 0, PS>), gs, bs, 0, s, uh, W, C, tw, xcd);
+    hipLaunchKernelGGL((k_yinv<T, N, 0, PS>), gs, bs, 0, s, W, C, tw, xcd);
+    if (check_launch("fused physical")) return -1;
+    const double sc = 1.0 / ((double)N * N * N);
+    if (u_out && z_c2r_rows(p, W, u_out, 3L * N * N, sc, s)) return -1;
+    if (curl_out && z_c2r_rows(p, W + 3 * C, curl_out, 3L * N * N, sc, s)) return -1;
+    return 0;
+  }
+};
+
+template <typename T, int N, class PS, class PX, class PZ>
+int step_impl(const sles_plan_s* p, int scheme, const sles_step_params* prm, void* uh_, void* uh0_, void* uh1_,
+              void* aux_, void* u_out, void* curl_out, double* scalars, void* work, hipStream_t s) {
+  using F = Fused<T, N, PS, PX, PZ>;
+  cpx<T>* uh = (cpx<T>*)uh_;
+  cpx<T>* uh0 = (cpx<T>*)uh0_;
+  cpx<T>* uh1 = (cpx<T>*)uh1_;
+  cpx<T>* aux = (cpx<T>*)aux_;
+  cpx<T>* W = (cpx<T>*)work;
+  const long C = ncplx(p);
+  const int les = prm->les_model;
+  const int nw = les ? 9 : 6;
+  double* part = (double*)(W + nw * C);
+  const bool ddt = prm->dt_device != 0;
+  StepArgs a{};
+  a.dt = prm->dt;
+  a.dt_dev = ddt ? scalars + 1 : nullptr;
+  a.nu = prm->nu;
+  a.forcing = prm->forcing;
+  const T coeff = (T)prm->sgs_coeff;
+  int nst = 1;
+  int epi = EPI_RHS;
+  switch (scheme) {
+    case SLES_SCHEME_RK4: nst = 4; epi = EPI_RK4; break;
+    case SLES_SCHEME_RK3: nst = 3; epi = EPI_RK3; break;
+    case SLES_SCHEME_ETD2: nst = 1; epi = EPI_ETD2; break;
+    case SLES_SCHEME_RHS: nst = 1; epi = EPI_RHS; break;
+    case SLES_SCHEME_NONLINEAR: nst = 1; epi = EPI_NONLIN; break;
+    default: return fail("sles_step: unknown scheme");
+  }
+  static const double rk4a[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
+  static const double rk4b[4] = {0.5, 0.5, 1.0, 0.0};
+  for (int st = 0; st < nst; ++st) {
+    a.scheme = epi;
+    a.stage = st;
+    a.ca = rk4a[st];
+    a.cb = rk4b[st];
+    double* pp = (ddt && st == 0) ? part : nullptr;
+    if (F::stage(p, les, coeff, uh, W, pp, s)) return -1;
+    if (pp) {
+      ProfScope ps_(PK_CFL, s);
+      hipLaunchKernelGGL(k_cfl, dim3(1), dim3(1024), 0, s, (const double*)part, F::ZB, prm->cfl, prm->dx, prm->nu,
+                         scalars);
+      if (check_launch("cfl")) return -1;
+    }
+    if (F::finish(p, les, uh, uh0, uh1, aux, W, a, s)) return -1;
+  }
+  if (u_out || curl_out) return F::physical(p, uh, W, u_out, curl_out, s);
+  return 0;
+}
+
+// Line plans selectable per kernel group (plan->tune_*): radix-8 default or
+// the radix-4 plan where one exists for N.
+template <int N>
+struct HasP4 {
+  static constexpr bool value = (N == 32 || N == 64 || N == 128 || N == 256 || N == 512);
+};
+
+template <typename T, int N>
+int step_t(const sles_plan_s* p, int scheme, const sles_step_params* prm, void* uh, void* uh0, void* uh1, void* aux,
+           void* u_out, void* curl_out, double* scalars, void* work, hipStream_t s) {
+  using D = LinePlan<N>;
+  if constexpr (HasP4<N>::value) {
+    using Q = LinePlan4<N>;
+    const int sel = (p->tune_strided ? 1 : 0) | (p->tune_xfwd ? 2 : 0) | (p->tune_z ? 4 : 0);
+#define SLES_SEL(K, A, B, Cz) \
+  case K:                     \
+    return step_impl<T, N, A, B, Cz>(p, scheme, prm, uh, uh0, uh1, aux, u_out, curl_out, scalars, work, s);
+    switch (sel) {
+      SLES_SEL(0, D, D, D)
+      SLES_SEL(1, Q, D, D)
+      SLES_SEL(2, D, Q, D)
+      SLES_SEL(3, Q, Q, D)
+      SLES_SEL(4, D, D, Q)
+      SLES_SEL(5, Q, D, Q)
+      SLES_SEL(6, D, Q, Q)
+      SLES_SEL(7, Q, Q, Q)
+    }
+#undef SLES_SEL
+  }
+  return step_impl<T, N, D, D, D>(p, scheme, prm, uh, uh0, uh1, aux, u_out, curl_out, scalars, work, s);
+}
+
+// Per-precision instantiation (sles_rhs_f64.hip / sles_rhs_f32.hip).
+int step_f64(const sles_plan_s* p, int scheme, const sles_step_params* prm, void* uh, void* uh0, void* uh1,
+             void* aux, void* u_out, void* curl_out, double* scalars, void* work, hipStream_t s);
+int step_f32(const sles_plan_s* p, int scheme, const sles_step_params* prm, void* uh, void* uh0, void* uh1,
+             void* aux, void* u_out, void* curl_out, double* scalars, void* work, hipStream_t s);
+
+}  // namespace sles

@@ -1,0 +1,19 @@
+// FP64 instantiation of the fused time-step kernels (sles_rhs.h).
+#include "sles_rhs.h"
+
+namespace sles {
+
+int step_f64(const sles_plan_s* p, int scheme, const sles_step_params* prm, void* uh, void* uh0, void* uh1,
+             void* aux, void* u_out, void* curl_out, double* scalars, void* work, hipStream_t s) {
+#define SLES_CASE(NN) \
+  case NN:            \
+    return step_t<double, NN>(p, scheme, prm, uh, uh0, uh1, aux, u_out, curl_out, scalars, work, s);
+  switch (p->n) {
+    SLES_FOR_EACH_N(SLES_CASE)
+    default:
+      return fail("sles_step: unsupported grid size");
+  }
+#undef SLES_CASE
+}
+
+}  // namespace sles

@@ -20,6 +20,7 @@ SLES_FP32 = 1
 SLES_LES_NONE = 0
 SLES_LES_SMAGORINSKY = 1
 SLES_LES_VREMAN = 2
+TUNE_KEYS = {"strided_radix4": 0, "xfwd_radix4": 1, "z_radix4": 2, "xcd": 3}
 
 _vp = ctypes.c_void_p
 _int = ctypes.c_int
@@ -33,6 +34,8 @@ _PROTOS = {
     "sles_plan_create": (_int, [ctypes.POINTER(_vp), _int, _int]),
     "sles_plan_destroy": (_int, [_vp]),
     "sles_plan_info": (_int, [_vp, ctypes.POINTER(_int), ctypes.POINTER(_int), ctypes.POINTER(_int)]),
+    "sles_plan_set_tuning": (_int, [_vp, _int, _int]),
+    "sles_plan_get_tuning": (_int, [_vp, _int, ctypes.POINTER(_int)]),
     "sles_fft_r2c": (_int, [_vp, _vp, _vp, _int, _dbl, _vp]),
     "sles_fft_c2r": (_int, [_vp, _vp, _vp, _vp, _int, _dbl, _vp]),
     "sles_cross_product": (_int, [_vp, _vp, _vp, _vp, _vp]),
@@ -53,6 +56,10 @@ _PROTOS = {
     "sles_spectrum": (_int, [_vp, _vp, _int, _vp, _vp, _vp, _vp]),
     "sles_scale": (_int, [_vp, _vp, _int, _dbl, _vp]),
     "sles_step_work_bytes": (_size, [_vp, _int]),
+    "sles_adjoint_work_bytes": (_size, [_vp, _int]),
+    "sles_adjoint_rhs": (_int, [_vp, _vp, _vp, _vp, _vp, _int, _int, _dbl, _vp, _vp, _vp]),
+    "sles_axpby": (_int, [_vp, _int, _dbl, _vp, _dbl, _vp, _vp]),
+    "sles_etd2_adjoint_combine": (_int, [_vp, _dbl, _dbl, _vp, _vp, _vp, _vp, _vp]),
     "sles_profile_enable": (_int, [_int]),
     "sles_profile_read": (_int, [_vp, _vp, _int]),
     "sles_step": (_int, [_vp, _int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
@@ -126,6 +133,18 @@ class Plan:
         with torch.cuda.device(self.device):
             call("sles_plan_create", ctypes.byref(h), self.n, precision)
         self.handle = h
+
+    def set_tuning(self, **kw):
+        for k, v in kw.items():
+            call("sles_plan_set_tuning", self.handle, TUNE_KEYS[k], int(v))
+
+    def get_tuning(self):
+        out = {}
+        for k, key in TUNE_KEYS.items():
+            v = ctypes.c_int()
+            call("sles_plan_get_tuning", self.handle, key, ctypes.byref(v))
+            out[k] = v.value
+        return out
 
     def __del__(self):
         h = getattr(self, "handle", None)

@@ -1,0 +1,248 @@
+"""Differentiable LES time integration: hand-written adjoint behind
+``torch.autograd.Function``.
+
+    solver = DifferentiableLES(N=256, nu=1e-4, dt=2e-3, nsteps=50, les_model="Vreman",
+                               scheme="ETD2", precision="fp32", checkpoint_every=10)
+    U_N = solver(U0, Cs)            # U0: (3, N, N, N/2+1) complex, Cs: 0-dim tensor
+    loss(U_N).backward()            # -> U0.grad, Cs.grad
+
+The forward pass runs the fused time steps (``sles_step``) and keeps the
+state every ``checkpoint_every`` steps.  The backward pass walks the
+segments in reverse: it recomputes the segment from its checkpoint (RK4:
+the four stage states of every step; ETD2: the state and nonlinear term of
+every step) and applies the transposed step, whose core is the hand-written
+vector-Jacobian product of the RHS (``sles_adjoint_rhs``, one fused
+five-pass launch sequence).  Gradients follow torch's convention for complex
+tensors (d/dRe + i d/dIm); d/dCs is accumulated in FP64 on the device.
+
+RK4 (src/RK4_method.py:20-35), for Y_s the stage states, k_s = f(Y_s):
+    mu_3 = dt a_3 lam;  lY_3 = J_3^T mu_3
+    mu_s = dt a_s lam + dt b_s lY_{s+1};  lY_s = J_s^T mu_s   (s = 2, 1, 0)
+    lam_n = lam + sum_s lY_s
+ETD2 (Cox-Matthews, U_{n+1} = E U_n + w1 F(U_n) - w0 F(U_{n-1})):
+    lam_n = E lam_{n+1} + J_F(U_n)^T (w1 lam_{n+1} - w0 lam_{n+2})
+with the first step an RK4 step whose F(U_0) also feeds the second step.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import math
+
+import torch
+
+from . import _lib, fused
+
+RK4_A = (1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0)
+RK4_B = (0.5, 0.5, 1.0)
+
+
+class DifferentiableLES:
+    def __init__(self, N, nu, dt, nsteps, les_model="Smagorinsky", scheme="RK4", precision="fp32",
+                 checkpoint_every=10, forcing=0.0, device="cuda"):
+        from .initialize_domain import InitializeDomain
+
+        if scheme not in ("RK4", "ETD2"):
+            raise ValueError("scheme must be 'RK4' or 'ETD2'")
+        kw = dict(LES_ON=1, LES_model=les_model, Cs=0.1) if les_model else {}
+        self.d = InitializeDomain(N=N, nu=nu, time_stepSize=dt, method=scheme, forcing=forcing,
+                                  precision=precision, device=device, **kw)
+        self.N, self.nu, self.dt, self.nsteps = N, nu, dt, int(nsteps)
+        self.scheme = scheme
+        self.K = max(1, int(checkpoint_every))
+        self.les_model = les_model
+        self.adj_work = torch.empty(int(_lib.lib.sles_adjoint_work_bytes(self.d.plan.handle, self.d.sgs_model)),
+                                    dtype=torch.uint8, device=self.d.device)
+        self.cs_grad = torch.zeros(1, dtype=torch.float64, device=self.d.device)
+
+    # -- coefficient and its Cs derivative -------------------------------------
+    def _set_cs(self, Cs):
+        self.d.Cs = float(Cs)
+
+    def dcoeff(self):
+        d = self.d
+        if not d.LES_ON:
+            return 0.0
+        if d.LES_model[0] == "S":
+            return 2.0 * d.Cs * d.dx * d.dx
+        return 5.0 * d.Cs * d.dx * d.dx
+
+    # -- primitives -------------------------------------------------------------
+    def rhs(self, Y, out):
+        d = self.d
+        d.U_hat.copy_(Y)
+        fused.rhs(d, out)
+
+    def nonlinear(self, Y, out):
+        d = self.d
+        d.U_hat.copy_(Y)
+        fused.nonlinear(d, out)
+
+    def vjp(self, Y, mu, out, accumulate=False, linear=1):
+        """out (+)= J(Y)^T mu; accumulates d/dCs into self.cs_grad."""
+        d = self.d
+        prm = fused.params(d, dt=self.dt)
+        _lib.call("sles_adjoint_rhs", d.plan.handle, ctypes.byref(prm), Y.data_ptr(), mu.data_ptr(),
+                  out.data_ptr(), 1 if accumulate else 0, int(linear), self.dcoeff(),
+                  self.cs_grad.data_ptr() if d.LES_ON else None, self.adj_work.data_ptr(), d.stream)
+
+    def axpby(self, a, x, b, y):
+        _lib.call("sles_axpby", self.d.plan.handle, 3, float(a), x.data_ptr(), float(b), y.data_ptr(), self.d.stream)
+
+    # -- forward ------------------------------------------------------------------
+    def _rk4_stages(self, U):
+        """Stage states Y_0..Y_3 of one RK4 step from U and the new state."""
+        Ys = [U]
+        k = torch.empty_like(U)
+        new = U.clone()
+        for s in range(4):
+            self.rhs(Ys[s], k)
+            self.axpby(self.dt * RK4_A[s], k, 1.0, new)
+            if s < 3:
+                Y = U.clone()
+                self.axpby(self.dt * RK4_B[s], k, 1.0, Y)
+                Ys.append(Y)
+        return Ys, new
+
+    def run_forward(self, U0, keep=None):
+        """Advance nsteps; returns the final state and the checkpoints
+        {step: (U, F_prev)}."""
+        d = self.d
+        ckpt = {}
+        d.U_hat.copy_(U0)
+        d.U_hat1.copy_(U0)
+        d.firstIteration = True
+        for n in range(self.nsteps):
+            if n % self.K == 0:
+                ckpt[n] = (d.U_hat.clone(), d.F0.clone() if (self.scheme == "ETD2" and n > 0) else None)
+            if self.scheme == "RK4":
+                fused.rk4_step(d, physical=False)
+            else:
+                fused.etd2_step(d, physical=False)
+        return d.U_hat.clone(), ckpt
+
+    # -- backward -----------------------------------------------------------------
+    def run_backward(self, ckpt, gU):
+        lam = gU.clone()
+        self.cs_grad.zero_()
+        if self.scheme == "RK4":
+            return self._backward_rk4(ckpt, lam)
+        return self._backward_etd2(ckpt, lam)
+
+    def _backward_rk4(self, ckpt, lam):
+        for start in sorted(ckpt, reverse=True):
+            end = min(start + self.K, self.nsteps)
+            U = ckpt[start][0].clone()
+            stages = []
+            for n in range(start, end):
+                Ys, U = self._rk4_stages(U)
+                stages.append(Ys)
+            for Ys in reversed(stages):
+                lam = self._rk4_step_adjoint(Ys, lam)
+        return lam
+
+    def _backward_etd2(self, ckpt, lam_final):
+        """lam_n = E lam_{n+1} + J_F(U_n)^T (w1 lam_{n+1} - w0 lam_{n+2})."""
+        d = self.d
+        N = self.nsteps
+        mu = torch.empty_like(lam_final)
+        # adjoints of the states U_{n+1} and U_{n+2} during the sweep
+        lam_next = lam_final.clone()  # lam_{n+1}
+        lam_next2 = None  # lam_{n+2}
+        for start in sorted(ckpt, reverse=True):
+            end = min(start + self.K, N)
+            # recompute U_start .. U_{end-1}
+            U, Fp = ckpt[start]
+            states = []
+            d.U_hat.copy_(U)
+            d.U_hat1.copy_(U)
+            if start == 0:
+                d.firstIteration = True
+            else:
+                d.F0.copy_(Fp)
+                d.firstIteration = False
+            for n in range(start, end):
+                states.append(d.U_hat.clone())
+                fused.etd2_step(d, physical=False)
+            for n in reversed(range(start, end)):
+                Un = states[n - start]
+                if n >= 1:
+                    lam_out = torch.empty_like(lam_next)
+                    _lib.call("sles_etd2_adjoint_combine", d.plan.handle, float(self.dt), float(self.nu),
+                              lam_next.data_ptr(), lam_next2.data_ptr() if lam_next2 is not None else None,
+                              mu.data_ptr(), lam_out.data_ptr(), d.stream)
+                    self.vjp(Un, mu, lam_out, accumulate=True, linear=2)
+                    lam_next2, lam_next = lam_next, lam_out
+                else:
+                    # step 0: RK4 bootstrap U_1 = RK4(U_0), plus F(U_0) in the step to U_2
+                    Ys, _ = self._rk4_stages(Un)
+                    lam0 = self._rk4_step_adjoint(Ys, lam_next)
+                    if lam_next2 is not None:
+                        # U_2 = ... - w0 F(U_0): contribution J_F(U_0)^T (-w0 lam_2)
+                        zero = torch.zeros_like(lam_next2)
+                        tmp = torch.empty_like(lam_next2)
+                        _lib.call("sles_etd2_adjoint_combine", d.plan.handle, float(self.dt), float(self.nu),
+                                  zero.data_ptr(), lam_next2.data_ptr(), mu.data_ptr(), tmp.data_ptr(), d.stream)
+                        self.vjp(Un, mu, lam0, accumulate=True, linear=2)
+                    lam_next2, lam_next = lam_next, lam0
+        return lam_next
+
+    def _rk4_step_adjoint(self, Ys, lam):
+        """lam_n from lam_{n+1} for one RK4 step with stage states Ys."""
+        lY = torch.empty_like(lam)
+        mu = lam.clone()
+        self.axpby(0.0, lam, self.dt * RK4_A[3], mu)  # mu = dt a_3 lam
+        self.vjp(Ys[3], mu, lY)  # lY = J_3^T mu
+        acc = lam.clone()
+        self.axpby(1.0, lY, 1.0, acc)
+        for s in (2, 1, 0):
+            mu.copy_(lam)
+            self.axpby(self.dt * RK4_B[s], lY, self.dt * RK4_A[s], mu)  # mu = dt b_s lY + dt a_s lam
+            self.vjp(Ys[s], mu, lY)
+            self.axpby(1.0, lY, 1.0, acc)
+        return acc
+
+    def __call__(self, U0, Cs):
+        return _Advance.apply(U0, Cs, self)
+
+
+class _Advance(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, U0, Cs, solver):
+        solver._set_cs(Cs.item() if torch.is_tensor(Cs) else Cs)
+        out, ckpt = solver.run_forward(U0.detach().to(solver.d.cdtype))
+        ctx.solver = solver
+        ctx.ckpt = ckpt
+        ctx.cs_dtype = Cs.dtype if torch.is_tensor(Cs) else torch.float64
+        ctx.cs_device = Cs.device if torch.is_tensor(Cs) else solver.d.device
+        ctx.in_dtype = U0.dtype
+        return out
+
+    @staticmethod
+    def backward(ctx, gU):
+        solver = ctx.solver
+        lam = solver.run_backward(ctx.ckpt, gU.detach().to(solver.d.cdtype).contiguous())
+        gCs = solver.cs_grad.clone().reshape(()).to(dtype=ctx.cs_dtype, device=ctx.cs_device)
+        return lam.to(ctx.in_dtype), gCs, None
+
+
+def shell_spectrum(U_hat, N, kmax=40):
+    """E(k) over unit shells |k| in [k - 1/2, k + 1/2), k = 1..kmax:
+    sum (1/2) w |U_hat|^2 / N^6 with the rfft doubling w (torch ops, differentiable)."""
+    dev = U_hat.device
+    k = torch.fft.fftfreq(N, 1.0 / N, dtype=torch.float64, device=dev)
+    kz = torch.arange(N // 2 + 1, dtype=torch.float64, device=dev)
+    kmag = torch.sqrt(k[:, None, None] ** 2 + k[None, :, None] ** 2 + kz[None, None, :] ** 2)
+    w = torch.full_like(kmag, 2.0)
+    w[..., 0] = 1.0
+    w[..., -1] = 1.0
+    e = 0.5 * w * (U_hat.real.double() ** 2 + U_hat.imag.double() ** 2).sum(0) / float(N) ** 6
+    idx = torch.floor(kmag + 0.5).to(torch.int64).flatten()
+    E = torch.zeros(int(idx.max()) + 1, dtype=torch.float64, device=dev).index_add(0, idx, e.flatten())
+    return E[1 : kmax + 1]
+
+
+def spectrum_loss(U_hat, target_E, N, kmax=40):
+    """Squared log-mismatch sum_{k <= kmax} (log E(k) - log E_target(k))^2."""
+    E = shell_spectrum(U_hat, N, kmax)
+    return ((torch.log(E) - torch.log(target_E[:kmax])) ** 2).sum()

@@ -1,0 +1,124 @@
+"""Autotuner for the fused time-step kernels.
+
+For one (N, precision, LES model) it times every combination of the kernel
+variants a plan exposes (``sles_plan_set_tuning``: radix-8 or radix-4 line
+plans for the strided passes, the x-forward/update pass and the z kernel, and
+the XCD-aware tile order) on a few RK4 steps of a synthetic field, and keeps
+the fastest.  Results persist in a JSON table (default: ``tuning.json`` next
+to this file, override with ``SLES_TUNING_TABLE``) that
+``InitializeDomain`` applies to its plan.
+
+    python -m sles.autotune --n 256 --precision fp64 --les smagorinsky
+
+Every variant computes the same arithmetic up to round-off order, so the
+choice never changes results beyond FP rounding (tests/test_gpu_fused.py
+checks each variant against the oracle).
+"""
+
+from __future__ import annotations
+
+import argparse
+import itertools
+import json
+import os
+import time
+
+import torch
+
+from . import _lib
+
+TABLE = os.environ.get("SLES_TUNING_TABLE", os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning.json"))
+KNOBS = ("strided_radix4", "xfwd_radix4", "z_radix4", "xcd")
+RADIX4_SIZES = (32, 64, 128, 256, 512)
+
+
+def key(N, precision, les_model):
+    return f"{int(N)}-{precision}-{int(les_model)}"
+
+
+def load_table(path=TABLE):
+    if not os.path.exists(path):
+        return {}
+    with open(path) as f:
+        return json.load(f)
+
+
+def apply(plan, N, precision, les_model, path=TABLE):
+    """Set the tuned variant of (N, precision, les_model) on ``plan`` if the
+    table has one; returns the settings applied (or None)."""
+    entry = load_table(path).get(key(N, precision, les_model))
+    if entry is None:
+        return None
+    plan.set_tuning(**{k: entry["settings"][k] for k in KNOBS if k in entry["settings"]})
+    return entry["settings"]
+
+
+def candidates(N):
+    radix = (0, 1) if N in RADIX4_SIZES else (0,)
+    for s, x, z, c in itertools.product(radix, radix, radix, (0, 1)):
+        yield dict(strided_radix4=s, xfwd_radix4=x, z_radix4=z, xcd=c)
+
+
+def time_variant(domain, settings, steps=3, warmup=1):
+    from . import fused
+
+    domain.plan.set_tuning(**settings)
+    for _ in range(warmup):
+        fused.rk4_step(domain, dt=1e-4, physical=True)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        fused.rk4_step(domain, dt=1e-4, physical=True)
+    e1.record()
+    e1.synchronize()
+    return e0.elapsed_time(e1) / steps
+
+
+def tune(N, precision="fp64", les="smagorinsky", steps=3, path=TABLE, verbose=True):
+    from .initial import isotropic_spectrum, set_spectral_state
+    from .initialize_domain import InitializeDomain
+
+    model = {"none": None, "smagorinsky": "Smagorinsky", "vreman": "Vreman"}[les.lower()]
+    kw = dict(LES_ON=1, LES_model=model, Cs=0.17) if model else {}
+    d = InitializeDomain(N=N, nu=1e-4, method="RK4", precision=precision, tuned=False, **kw)
+    set_spectral_state(d, isotropic_spectrum(N, seed=0, dtype=d.cdtype))
+    results = []
+    for st in candidates(N):
+        ms = time_variant(d, st, steps=steps)
+        results.append((ms, st))
+        if verbose:
+            print(f"[autotune] N={N} {precision} {les}: {st} -> {ms:.3f} ms/step", flush=True)
+    results.sort(key=lambda r: r[0])
+    best_ms, best = results[0]
+    table = load_table(path)
+    table[key(N, precision, d.sgs_model)] = {
+        "settings": best,
+        "ms_per_step": round(best_ms, 4),
+        "default_ms_per_step": round(next(ms for ms, st in results if not any(st.values())), 4),
+        "device": torch.cuda.get_device_name(),
+        "tuned_at": time.strftime("%Y-%m-%d %H:%M:%S"),
+    }
+    with open(path, "w") as f:
+        json.dump(table, f, indent=1, sort_keys=True)
+    return best, best_ms, results
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--n", type=int, nargs="+", default=[256])
+    ap.add_argument("--precision", nargs="+", default=["fp64", "fp32"])
+    ap.add_argument("--les", nargs="+", default=["smagorinsky"])
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--table", default=TABLE)
+    a = ap.parse_args(argv)
+    for N in a.n:
+        for prec in a.precision:
+            for les in a.les:
+                best, ms, _ = tune(N, prec, les, steps=a.steps, path=a.table)
+                print(f"[autotune] best N={N} {prec} {les}: {best} {ms:.3f} ms/step", flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -74,6 +74,7 @@ class InitializeDomain:
         device=None,
         fused: bool = True,
         etd2_form: str = "cox_matthews",
+        tuned: bool = True,
     ):
         if dimension != 3:
             raise NotImplementedError("only the 3D solver is implemented (the 2D branch is out of scope)")
@@ -156,6 +157,12 @@ class InitializeDomain:
             self.aTemp = rz(N, N, N)
         else:
             self.LES_model = None
+        # kernel variants chosen by the autotuner for this (N, precision, model)
+        self.tuning = None
+        if tuned:
+            from . import autotune
+
+            self.tuning = autotune.apply(self.plan, N, precision, self.sgs_model)
         # device scalars of the fused path (u_max, dt) and reduction scratch
         self._scalars = torch.zeros(8, dtype=torch.float64, device=self.device)
         self._red_work = torch.empty(int(_lib.lib.sles_reduce_work_bytes()) // 8, dtype=torch.float64,

@@ -1,0 +1,171 @@
+"""Hand-written adjoint: the VJP kernel against finite differences of the
+forward RHS kernel (dot-product test), d/dCs against finite differences, and
+whole RK4 / ETD2 trajectories against an FP64 torch-autograd reference
+(tests/torch_reference.py, CPU) and central finite differences (h = 1e-3
+for Cs, as in the differentiable-LES configuration)."""
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+import torch_reference as TR
+from helpers import random_field
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _need_gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+
+
+def rdot(a, b):
+    """Real inner product of complex arrays viewed as (Re, Im) vectors."""
+    return float((torch.conj(a) * b).real.sum())
+
+
+def _solver(N, les, scheme="RK4", nsteps=3, dt=0.01, nu=0.01, precision="fp64", K=2, forcing=0.0):
+    from sles.adjoint import DifferentiableLES
+
+    return DifferentiableLES(N=N, nu=nu, dt=dt, nsteps=nsteps, les_model=les, scheme=scheme, precision=precision,
+                             checkpoint_every=K, forcing=forcing)
+
+
+def _state(N, seed, k_cut=3.0):
+    u = random_field(N, seed, k_cut=k_cut)
+    U = np.fft.rfftn(u, axes=(1, 2, 3))
+    return torch.as_tensor(U, device="cuda")
+
+
+@pytest.mark.parametrize("les", [None, "Smagorinsky", "Vreman"])
+@pytest.mark.parametrize("N", [16, 24])
+@pytest.mark.parametrize("forcing", [0.0, 0.1])
+def test_vjp_dot_product(les, N, forcing):
+    """<J(Y) v, mu> = <v, J(Y)^T mu> with J v by central differences."""
+    s = _solver(N, les, forcing=forcing)
+    s._set_cs(0.17)
+    Y = _state(N, 1)
+    g = torch.Generator(device="cuda").manual_seed(2)
+    v = torch.complex(torch.randn(Y.shape, generator=g, device="cuda", dtype=torch.float64),
+                      torch.randn(Y.shape, generator=g, device="cuda", dtype=torch.float64))
+    mu = torch.complex(torch.randn(Y.shape, generator=g, device="cuda", dtype=torch.float64),
+                       torch.randn(Y.shape, generator=g, device="cuda", dtype=torch.float64))
+    eps = 1e-6 * float(Y.abs().max()) / float(v.abs().max())
+    fp = torch.empty_like(Y)
+    fm = torch.empty_like(Y)
+    s.rhs(Y + eps * v, fp)
+    s.rhs(Y - eps * v, fm)
+    Jv = (fp - fm) / (2 * eps)
+    JTmu = torch.empty_like(Y)
+    s.vjp(Y, mu, JTmu)
+    lhs, rhs = rdot(mu, Jv), rdot(JTmu, v)
+    assert abs(lhs - rhs) < 1e-6 * max(abs(lhs), abs(rhs)), (lhs, rhs)
+
+
+@pytest.mark.parametrize("les", ["Smagorinsky", "Vreman"])
+def test_vjp_cs_derivative(les):
+    N = 16
+    s = _solver(N, les)
+    Y = _state(N, 3)
+    g = torch.Generator(device="cuda").manual_seed(4)
+    mu = torch.complex(torch.randn(Y.shape, generator=g, device="cuda", dtype=torch.float64),
+                       torch.randn(Y.shape, generator=g, device="cuda", dtype=torch.float64))
+    Cs, h = 0.15, 1e-5
+    out = torch.empty_like(Y)
+
+    def f(c):
+        s._set_cs(c)
+        s.rhs(Y, out)
+        return rdot(mu, out)
+
+    fd = (f(Cs + h) - f(Cs - h)) / (2 * h)
+    s._set_cs(Cs)
+    s.cs_grad.zero_()
+    s.vjp(Y, mu, out)
+    got = float(s.cs_grad.item())
+    assert abs(got - fd) < 1e-6 * abs(fd), (got, fd)
+
+
+def _torch_reference_grads(U0, Cs, N, nu, dt, nsteps, les, target):
+    """FP64 autograd on CPU through torch_reference.rk4_step."""
+    U = U0.detach().cpu().clone().requires_grad_(True)
+    c = torch.tensor(Cs, dtype=torch.float64, requires_grad=True)
+    X = U
+    for _ in range(nsteps):
+        X = TR.rk4_step(X, c, N, nu, dt, les)
+    E = TR.shell_spectrum(X, N, kmax=target.numel())
+    loss = ((torch.log(E) - torch.log(target)) ** 2).sum()
+    loss.backward()
+    return float(loss), U.grad, float(c.grad)
+
+
+@pytest.mark.parametrize("les", ["Smagorinsky", "Vreman"])
+def test_rk4_adjoint_vs_autograd_and_fd(les):
+    from sles.adjoint import spectrum_loss
+
+    N, nu, dt, nsteps = 16, 0.01, 0.01, 4
+    U0 = _state(N, 7)
+    target = TR.shell_spectrum(torch.as_tensor(_state(N, 8).cpu()), N, kmax=6) * 1.3
+    Cs = 0.17
+    ref_loss, ref_gU, ref_gCs = _torch_reference_grads(U0, Cs, N, nu, dt, nsteps, les, target)
+
+    s = _solver(N, les, nsteps=nsteps, dt=dt, nu=nu, K=3)
+    U = U0.clone().requires_grad_(True)
+    c = torch.tensor(Cs, dtype=torch.float64, device="cuda", requires_grad=True)
+    loss = spectrum_loss(s(U, c), target.cuda(), N, kmax=6)
+    loss.backward()
+    assert math.isclose(float(loss), ref_loss, rel_tol=1e-10)
+    gU = U.grad.cpu()
+    assert float((gU - ref_gU).abs().max()) < 1e-8 * float(ref_gU.abs().max())
+    assert math.isclose(float(c.grad), ref_gCs, rel_tol=1e-8)
+    # central finite differences in Cs, h = 1e-3
+    h = 1e-3
+
+    def L(cv):
+        with torch.no_grad():
+            return float(spectrum_loss(s(U0, torch.tensor(cv, dtype=torch.float64)), target.cuda(), N, kmax=6))
+
+    fd = (L(Cs + h) - L(Cs - h)) / (2 * h)
+    assert abs(float(c.grad) - fd) < 1e-4 * abs(fd) + 1e-12
+    # directional finite difference in U0
+    g = torch.Generator(device="cuda").manual_seed(9)
+    v = torch.complex(torch.randn(U0.shape, generator=g, device="cuda", dtype=torch.float64),
+                      torch.randn(U0.shape, generator=g, device="cuda", dtype=torch.float64))
+    v *= float(U0.abs().max()) * 1e-6 / float(v.abs().max())
+    with torch.no_grad():
+        Lp = float(spectrum_loss(s(U0 + v, torch.tensor(Cs)), target.cuda(), N, kmax=6))
+        Lm = float(spectrum_loss(s(U0 - v, torch.tensor(Cs)), target.cuda(), N, kmax=6))
+    dd = (Lp - Lm) / 2
+    assert abs(rdot(U.grad, v) - dd) < 1e-5 * abs(dd)
+
+
+@pytest.mark.parametrize("les", ["Vreman", "Smagorinsky"])
+def test_etd2_adjoint_vs_fd(les):
+    from sles.adjoint import spectrum_loss
+
+    N, nu, dt, nsteps = 16, 0.01, 0.005, 6
+    U0 = _state(N, 11)
+    target = TR.shell_spectrum(torch.as_tensor(_state(N, 12).cpu()), N, kmax=6)
+    s = _solver(N, les, scheme="ETD2", nsteps=nsteps, dt=dt, nu=nu, K=4)
+    Cs = 0.08 if les == "Vreman" else 0.17
+    U = U0.clone().requires_grad_(True)
+    c = torch.tensor(Cs, dtype=torch.float64, device="cuda", requires_grad=True)
+    loss = spectrum_loss(s(U, c), target.cuda(), N, kmax=6)
+    loss.backward()
+    h = 1e-3
+
+    def L(U_, cv):
+        with torch.no_grad():
+            return float(spectrum_loss(s(U_, torch.tensor(cv, dtype=torch.float64)), target.cuda(), N, kmax=6))
+
+    fd = (L(U0, Cs + h) - L(U0, Cs - h)) / (2 * h)
+    assert abs(float(c.grad) - fd) < 1e-4 * abs(fd) + 1e-12
+    g = torch.Generator(device="cuda").manual_seed(13)
+    v = torch.complex(torch.randn(U0.shape, generator=g, device="cuda", dtype=torch.float64),
+                      torch.randn(U0.shape, generator=g, device="cuda", dtype=torch.float64))
+    v *= float(U0.abs().max()) * 1e-6 / float(v.abs().max())
+    dd = (L(U0 + v, Cs) - L(U0 - v, Cs)) / 2
+    assert abs(rdot(U.grad, v) - dd) < 1e-5 * abs(dd)

@@ -146,3 +146,24 @@ def test_taylor_green_64_rk4_trajectory():
         O.rk4_step(o)
     assert rel(torch.stack(g.U), np.array(o.U)) < 1e-10
     assert rel(g.U_hat, o.U_hat) < 1e-10
+
+
+@pytest.mark.parametrize("les", [None, "Smagorinsky", "Vreman"])
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_every_tuning_variant_matches_oracle(les, precision):
+    """Each kernel variant the autotuner may pick gives the oracle's step."""
+    from sles import autotune, fused
+
+    N = 32
+    tol = 1e-11 if precision == "fp64" else 1e-5
+    g, o, u = twins(N, seed=14, fused=True, precision=precision,
+                    **_kw(les, nu=0.005, time_stepSize=0.01, method="RK4"))
+    O.rk4_step(o)
+    start = g.U_hat.clone()
+    for st in autotune.candidates(N):
+        g.U_hat.copy_(start)
+        g.U_hat1.copy_(start)
+        g.plan.set_tuning(**st)
+        assert g.plan.get_tuning() == st
+        fused.rk4_step(g)
+        assert rel(g.U_hat, o.U_hat) < tol, st

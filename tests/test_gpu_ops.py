@@ -178,7 +178,8 @@ def test_reductions_and_spectrum():
     S.calculate_Ek(g, Eg)
     O.calculate_Ek(o, Eo)
     np.testing.assert_allclose(Eg.temp_k, Eo.temp_k, rtol=1e-13)
-    np.testing.assert_allclose(Eg.temp_Ek, Eo.temp_Ek, rtol=1e-11)
+    # bins far down the spectrum sit at round-off of the total: compare against the peak
+    np.testing.assert_allclose(Eg.temp_Ek, Eo.temp_Ek, rtol=1e-11, atol=1e-14 * Eo.temp_Ek.max())
 
 
 def test_fp32_ops_close_to_fp64():
