@@ -542,7 +542,8 @@ int adjoint_impl(const sles_plan_s* p, const sles_step_params* prm, const void* 
   const int xcd = p->tune_xcd;
   cpx<T>* A = (cpx<T>*)work;
   double* part = (double*)(A + 17 * C);
-  const dim3 gs((unsigned)((C + SS::TK - 1) / SS::TK)), bs(SS::THREADS);
+  constexpr long NCOL = (long)N * (N / 2 + 1);  // columns of every strided pass
+  const dim3 gs((unsigned)((NCOL + SS::TK - 1) / SS::TK)), bs(SS::THREADS);
   const T scale = T(1.0 / ((double)N * N * N));
   hipLaunchKernelGGL((k_adj_xinv<T, N, PS>), gs, bs, 0, s, (const cpx<T>*)y, (const cpx<T>*)mu, A, C, tw, xcd);
   hipLaunchKernelGGL((k_adj_yinv<T, N, PS>), gs, bs, 0, s, A, C, tw, xcd);

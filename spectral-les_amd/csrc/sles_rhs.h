@@ -749,7 +749,8 @@ struct Fused {
     const long C = ncplx(p);
     const cpx<T>* tw = (const cpx<T>*)p->tw;
     const int xcd = p->tune_xcd;
-    const dim3 gs((unsigned)((C + SS::TK - 1) / SS::TK)), bs(SS::THREADS);
+    constexpr long NCOL = (long)N * (N / 2 + 1);  // columns of every strided pass
+    const dim3 gs((unsigned)((NCOL + SS::TK - 1) / SS::TK)), bs(SS::THREADS);
     const T scale = T(1.0 / ((double)N * N * N));
     if (les) {
       { ProfScope ps_(PK_XINV, s); hipLaunchKernelGGL((k_xinv<T, N, 1, PS>), gs, bs, 0, s, uh, W, C, tw, xcd); }
@@ -775,8 +776,9 @@ struct Fused {
     const long C = ncplx(p);
     const cpx<T>* tw = (const cpx<T>*)p->tw;
     const int xcd = p->tune_xcd;
-    const dim3 gs((unsigned)((C + SS::TK - 1) / SS::TK)), bs(SS::THREADS);
-    const dim3 gx((unsigned)((C + SX::TK - 1) / SX::TK)), bx(SX::THREADS);
+    constexpr long NCOL = (long)N * (N / 2 + 1);  // columns of every strided pass
+    const dim3 gs((unsigned)((NCOL + SS::TK - 1) / SS::TK)), bs(SS::THREADS);
+    const dim3 gx((unsigned)((NCOL + SX::TK - 1) / SX::TK)), bx(SX::THREADS);
     if (les) {
       { ProfScope ps_(PK_YFWD, s); hipLaunchKernelGGL((k_yfwd<T, N, 1, PS>), gs, bs, 0, s, W, C, tw, xcd); }
       { ProfScope ps_(PK_XFWD, s);
@@ -793,7 +795,8 @@ struct Fused {
     const long C = ncplx(p);
     const cpx<T>* tw = (const cpx<T>*)p->tw;
     const int xcd = p->tune_xcd;
-    const dim3 gs((unsigned)((C + SS::TK - 1) / SS::TK)), bs(SS::THREADS);
+    constexpr long NCOL = (long)N * (N / 2 + 1);  // columns of every strided pass
+    const dim3 gs((unsigned)((NCOL + SS::TK - 1) / SS::TK)), bs(SS::THREADS);
     ProfScope ps_(PK_PHYS, s);
     hipLaunchKernelGGL((k_xinv<T, N, 0, PS>), gs, bs, 0, s, uh, W, C, tw, xcd);
     hipLaunchKernelGGL((k_yinv<T, N, 0, PS>), gs, bs, 0, s, W, C, tw, xcd);
