@@ -146,6 +146,22 @@ def run_gpu(N, precision, steps, warmup, profile_steps=3):
     return out
 
 
+def measured_traffic(kernel, N, precision):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (profiles/hbm_traffic_rk4_smagorinsky_<N>_<precision>.txt,
+    scaled by the plain-copy calibration recorded there), or None."""
+    f = os.path.join(ROOT, "profiles", f"hbm_traffic_rk4_smagorinsky_{N}_{precision}.txt")
+    key = f"traffic_bytes_per_launch {kernel} {precision} {N}:"
+    try:
+        with open(f) as fh:
+            for line in fh:
+                if line.startswith(key):
+                    return int(line[len(key):].strip())
+    except OSError:
+        return None
+    return None
+
+
 def roofline_of(res, N, precision, les=True):
     R, C = field_bytes(N, precision)
     kern = res["kernels"]
@@ -176,7 +192,7 @@ def roofline_of(res, N, precision, les=True):
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": None,
+        "traffic": measured_traffic(best, N, precision),
         "algorithmic_bytes_per_launch": int(nbytes),
         "avg_launch_ms": round(avg_s * 1e3, 4),
     }, per_kernel

@@ -242,13 +242,12 @@ __global__ void __launch_bounds__(ZLines<PL>::THREADS)
   constexpr int LPB = ZLines<PL>::LPB;
   using IFFT = LineFFTP<T, PL, +1>;
   using FFT = LineFFTP<T, PL, -1>;
-  __shared__ cpx<T> lds[LPB * N];
+  __shared__ cpx<T> lds[LPB * LdsLine<T>::stride(N)];
   const int line = threadIdx.x / NT, t = threadIdx.x % NT;
   const long row = (long)blockIdx.x * LPB + line;
   const bool valid = row < (long)N * N;
   const long base = (valid ? row : 0) * NH;
-  cpx<T>* lrow = lds + line * N;
-  const LdsLinear map{line * N};
+  const LdsLine<T> map{line * LdsLine<T>::stride(N)};
   cpx<T>* w[17];
 #pragma unroll
   for (int i = 0; i < 17; ++i) w[i] = A + i * C + base;
@@ -394,7 +393,7 @@ __global__ void __launch_bounds__(ZLines<PL>::THREADS)
 #define SLES_FWD(QA, QB)                                               \
   _Pragma("unroll") for (int m = 0; m < P; ++m) v[m] = {QA[m], QB[m]}; \
   FFT::run(v, lds, map, t, tw);                                        \
-  split_pair<T, N, P, NT>(v, lrow, t, Ah, Bh);
+  split_pair<T, N, P, NT>(v, lds, map, t, Ah, Bh);
   SLES_FWD(a[2][0], a[2][1])
 #pragma unroll
   for (int m = 0; m < KH; ++m) {

@@ -183,10 +183,15 @@ struct Butterfly<T, 8, DIR> {
 // ---------------------------------------------------------------------------
 // LDS index maps used for the exchange between stages.
 // ---------------------------------------------------------------------------
-// Lines stored contiguously (contiguous-axis passes): position p of the line.
-struct LdsLinear {
+// Lines stored contiguously (contiguous-axis passes): position p of the line,
+// with one pad element after every 256 bytes so that the stride-R writes of
+// an exchange spread over all LDS banks.  A line occupies stride(N) elements.
+template <typename T>
+struct LdsLine {
+  static constexpr int S = sizeof(T) == 8 ? 4 : 5;  // 16 (FP64) / 32 (FP32) complex per 256 B
+  static constexpr int stride(int n) { return n + (n >> S); }
   int base;
-  __device__ __forceinline__ int operator()(int p) const { return base + p; }
+  __device__ __forceinline__ int operator()(int p) const { return base + p + (p >> S); }
 };
 // Columns interleaved (strided passes): position p of column c, TK columns.
 template <int TK>

@@ -43,7 +43,8 @@ __global__ void __launch_bounds__(ZShape<N>::THREADS)
             const cpx<T>* __restrict__ tw) {
   using S = ZShape<N>;
   constexpr int P = S::P, NT = S::NT, NH = S::NH;
-  __shared__ cpx<T> lds[S::LPB * N];
+  __shared__ cpx<T> lds[S::LPB * LdsLine<T>::stride(N)];
+  const LdsLine<T> map{(int)(threadIdx.x / S::NT) * LdsLine<T>::stride(N)};
   const int line = threadIdx.x / NT, t = threadIdx.x % NT;
   const long ra = 2 * ((long)blockIdx.x * S::LPB + line), rb = ra + 1;
   const bool va = ra < nrows, vb = rb < nrows;
@@ -56,17 +57,17 @@ __global__ void __launch_bounds__(ZShape<N>::THREADS)
     v[m].x = va ? xa : T(0);
     v[m].y = vb ? xb : T(0);
   }
-  LineFFT<T, N, -1>::run(v, lds, LdsLinear{line * N}, t, tw);
+  LineFFT<T, N, -1>::run(v, lds, map, t, tw);
   // Split Z = FFT(a + i b):  A_k = (Z_k + conj Z_{N-k})/2,  B_k = (Z_k - conj Z_{N-k})/(2i)
 #pragma unroll
-  for (int m = 0; m < P; ++m) lds[line * N + t + m * NT] = v[m];
+  for (int m = 0; m < P; ++m) lds[map(t + m * NT)] = v[m];
   block_sync();
 #pragma unroll
   for (int m = 0; m <= P / 2; ++m) {
     const int k = t + m * NT;
     if (k <= N / 2) {
       const cpx<T> zk = v[m];
-      const cpx<T> zn = conj(lds[line * N + ((N - k) % N)]);
+      const cpx<T> zn = conj(lds[map((N - k) % N)]);
       const cpx<T> a = T(0.5) * (zk + zn);
       const cpx<T> d = zk - zn;
       const cpx<T> b = {T(0.5) * d.y, T(-0.5) * d.x};
@@ -82,7 +83,8 @@ __global__ void __launch_bounds__(ZShape<N>::THREADS)
             const cpx<T>* __restrict__ tw) {
   using S = ZShape<N>;
   constexpr int P = S::P, NT = S::NT, NH = S::NH;
-  __shared__ cpx<T> lds[S::LPB * N];
+  __shared__ cpx<T> lds[S::LPB * LdsLine<T>::stride(N)];
+  const LdsLine<T> map{(int)(threadIdx.x / S::NT) * LdsLine<T>::stride(N)};
   const int line = threadIdx.x / NT, t = threadIdx.x % NT;
   const long ra = 2 * ((long)blockIdx.x * S::LPB + line), rb = ra + 1;
   const bool va = ra < nrows, vb = rb < nrows;
@@ -104,7 +106,7 @@ __global__ void __launch_bounds__(ZShape<N>::THREADS)
     }
     v[m] = {a.x - b.y, a.y + b.x};  // a + i b
   }
-  LineFFT<T, N, +1>::run(v, lds, LdsLinear{line * N}, t, tw);
+  LineFFT<T, N, +1>::run(v, lds, map, t, tw);
 #pragma unroll
   for (int m = 0; m < P; ++m) {
     const int z = t + m * NT;

@@ -229,17 +229,17 @@ __device__ __forceinline__ void load_pair(cpx<T> (&v)[P], const cpx<T>* A, bool 
 // After a forward FFT of a + i b held in v (natural layout), split into the
 // half-spectra A_k, B_k of a and b for this thread's bins k = t + m*NT <= N/2,
 // m = 0..P/2.  Uses the line's LDS row; ends with a barrier.
-template <typename T, int N, int P, int NT>
-__device__ __forceinline__ void split_pair(const cpx<T> (&v)[P], cpx<T>* row, int t, cpx<T> (&A)[P / 2 + 1],
-                                           cpx<T> (&B)[P / 2 + 1]) {
+template <typename T, int N, int P, int NT, class Map>
+__device__ __forceinline__ void split_pair(const cpx<T> (&v)[P], cpx<T>* lds, const Map& map, int t,
+                                           cpx<T> (&A)[P / 2 + 1], cpx<T> (&B)[P / 2 + 1]) {
 #pragma unroll
-  for (int m = 0; m < P; ++m) row[t + m * NT] = v[m];
+  for (int m = 0; m < P; ++m) lds[map(t + m * NT)] = v[m];
   block_sync();
 #pragma unroll
   for (int m = 0; m <= P / 2; ++m) {
     const int k = t + m * NT;
     const cpx<T> zk = v[m];
-    const cpx<T> zn = conj(row[(N - k) % N]);
+    const cpx<T> zn = conj(lds[map((N - k) % N)]);
     const cpx<T> d = zk - zn;
     A[m] = T(0.5) * (zk + zn);
     B[m] = {T(0.5) * d.y, T(-0.5) * d.x};
@@ -276,13 +276,12 @@ __global__ void __launch_bounds__(ZLines<PL>::THREADS)
   constexpr int LPB = ZLines<PL>::LPB;
   using IFFT = LineFFTP<T, PL, +1>;
   using FFT = LineFFTP<T, PL, -1>;
-  __shared__ cpx<T> lds[LPB * N];
+  __shared__ cpx<T> lds[LPB * LdsLine<T>::stride(N)];
   const int line = threadIdx.x / NT, t = threadIdx.x % NT;
   const long row = (long)blockIdx.x * LPB + line;
   const bool valid = row < (long)N * N;
   const long base = (valid ? row : 0) * NH;
-  cpx<T>* lrow = lds + line * N;
-  const LdsLinear map{line * N};
+  const LdsLine<T> map{line * LdsLine<T>::stride(N)};
   const cpx<T>* w[9];
 #pragma unroll
   for (int i = 0; i < 9; ++i) w[i] = W + i * C + base;
@@ -351,7 +350,7 @@ __global__ void __launch_bounds__(ZLines<PL>::THREADS)
 #define SLES_FWD(QA, QB)                                     \
   _Pragma("unroll") for (int m = 0; m < P; ++m) v[m] = {QA[m], QB[m]}; \
   FFT::run(v, lds, map, t, tw);                              \
-  split_pair<T, N, P, NT>(v, lrow, t, A, B);
+  split_pair<T, N, P, NT>(v, lds, map, t, A, B);
   SLES_FWD(q4, q5)  // tau_xy, tau_xz
 #pragma unroll
   for (int m = 0; m < KH; ++m) {
@@ -402,13 +401,12 @@ __global__ void __launch_bounds__(ZLines<PL>::THREADS)
   constexpr int LPB = ZLines<PL>::LPB;
   using IFFT = LineFFTP<T, PL, +1>;
   using FFT = LineFFTP<T, PL, -1>;
-  __shared__ cpx<T> lds[LPB * N];
+  __shared__ cpx<T> lds[LPB * LdsLine<T>::stride(N)];
   const int line = threadIdx.x / NT, t = threadIdx.x % NT;
   const long row = (long)blockIdx.x * LPB + line;
   const bool valid = row < (long)N * N;
   const long base = (valid ? row : 0) * NH;
-  cpx<T>* lrow = lds + line * N;
-  const LdsLinear map{line * N};
+  const LdsLine<T> map{line * LdsLine<T>::stride(N)};
   cpx<T>* w[6];
 #pragma unroll
   for (int i = 0; i < 6; ++i) w[i] = W + i * C + base;
@@ -439,7 +437,7 @@ __global__ void __launch_bounds__(ZLines<PL>::THREADS)
 #pragma unroll
   for (int m = 0; m < P; ++m) v[m] = {n0[m], n1[m]};
   FFT::run(v, lds, map, t, tw);
-  split_pair<T, N, P, NT>(v, lrow, t, A, B);
+  split_pair<T, N, P, NT>(v, lds, map, t, A, B);
 #pragma unroll
   for (int m = 0; m < KH; ++m)
     if (valid && t + m * NT <= N / 2) {
@@ -449,7 +447,7 @@ __global__ void __launch_bounds__(ZLines<PL>::THREADS)
 #pragma unroll
   for (int m = 0; m < P; ++m) v[m] = {n2[m], T(0)};
   FFT::run(v, lds, map, t, tw);
-  split_pair<T, N, P, NT>(v, lrow, t, A, B);
+  split_pair<T, N, P, NT>(v, lds, map, t, A, B);
 #pragma unroll
   for (int m = 0; m < KH; ++m)
     if (valid && t + m * NT <= N / 2) w[2][t + m * NT] = A[m];

@@ -88,6 +88,69 @@ def precision_study(N=256, checkpoints=(100,), seed=1, les="Smagorinsky", Cs=0.1
     return out
 
 
+def spectral_cut(U_hat, N_from, N_to):
+    """Keep |kx|, |ky| < N_to/2 and kz <= N_to/2 of an N_from^3 spectrum, scaled
+    by (N_to/N_from)^3 for the unscaled forward-transform convention."""
+    h = N_to // 2
+    sel = torch.cat([torch.arange(0, h), torch.arange(N_from - h, N_from)]).to(U_hat.device)
+    out = U_hat[:, sel][:, :, sel][..., : h + 1].clone()
+    out[:, h] = 0  # kx = -N_to/2 (|kx| < N_to/2 kept)
+    out[:, :, h] = 0
+    return out * (float(N_to) / float(N_from)) ** 3
+
+
+def differentiable_study(N=256, N_target=512, steps=50, target_steps=50, dt=2e-3, nu=1e-4, c_vreman=0.08,
+                         checkpoint_every=10, seed=2, fd_h=1e-3):
+    """Vreman LES, FP32, ETD2, dt = 2e-3 over a `steps` window; loss = squared
+    log-mismatch of E(k), k <= 40, against a Smagorinsky N_target^3 run cut to N^3."""
+    from .adjoint import DifferentiableLES, spectrum_loss
+
+    # target spectrum
+    t = InitializeDomain(N=N_target, nu=nu, method="RK4", LES_ON=1, LES_model="Smagorinsky", Cs=0.17,
+                         precision="fp32")
+    set_spectral_state(t, isotropic_spectrum(N_target, seed=seed + 100, dtype=t.cdtype))
+    for _ in range(target_steps):
+        fused.rk4_step(t, dt=dt, physical=False)
+    Ucut = spectral_cut(t.U_hat.to(torch.complex128), N_target, N)
+    del t
+    torch.cuda.empty_cache()
+    target = shell_energy(Ucut, N)[1:41].clone()
+    del Ucut
+    solver = DifferentiableLES(N=N, nu=nu, dt=dt, nsteps=steps, les_model="Vreman", scheme="ETD2",
+                               precision="fp32", checkpoint_every=checkpoint_every)
+    # Vreman coefficient c = 2.5 Cs^2  ->  Cs = sqrt(c / 2.5)
+    cs0 = math.sqrt(c_vreman / 2.5)
+    U0 = isotropic_spectrum(N, seed=seed, dtype=torch.complex64)
+    U = U0.clone().requires_grad_(True)
+    c = torch.tensor(cs0, dtype=torch.float64, device="cuda", requires_grad=True)
+    torch.cuda.synchronize()
+    a = time.perf_counter()
+    out = solver(U, c)
+    torch.cuda.synchronize()
+    b = time.perf_counter()
+    loss = spectrum_loss(out, target, N, kmax=40)
+    loss.backward()
+    torch.cuda.synchronize()
+    e = time.perf_counter()
+
+    def L(cv):
+        with torch.no_grad():
+            return float(spectrum_loss(solver(U0, torch.tensor(cv, dtype=torch.float64)), target, N, kmax=40))
+
+    fd = (L(cs0 + fd_h) - L(cs0 - fd_h)) / (2 * fd_h)
+    res = {
+        "N": N, "steps": steps, "dt": dt, "scheme": "ETD2", "les_model": "Vreman", "precision": "fp32",
+        "c_vreman": c_vreman, "Cs": cs0, "checkpoint_every": checkpoint_every,
+        "loss": float(loss), "dloss_dCs_adjoint": float(c.grad), "dloss_dCs_central_fd_h1e-3": fd,
+        "dloss_dc_vreman_adjoint": float(c.grad) / (5.0 * cs0),
+        "grad_U0_norm": float(U.grad.abs().pow(2).sum().sqrt()),
+        "forward_s": b - a, "backward_s": e - b,
+        "backward_over_forward": (e - b) / (b - a),
+    }
+    print(json.dumps(res), flush=True)
+    return res
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -96,7 +159,34 @@ def main(argv=None):
     p.add_argument("--steps", type=int, nargs="+", default=[100])
     p.add_argument("--les", default="Smagorinsky")
     p.add_argument("--out", default=None)
+    q = sub.add_parser("differentiable")
+    q.add_argument("--n", type=int, default=256)
+    q.add_argument("--n-target", type=int, default=512)
+    q.add_argument("--steps", type=int, default=50)
+    q.add_argument("--out", default=None)
+    r = sub.add_parser("ensemble")
+    r.add_argument("--steps", type=int, default=200)
+    r.add_argument("--out", default=None)
     a = ap.parse_args(argv)
+    if a.cmd == "differentiable":
+        res = differentiable_study(a.n, a.n_target, a.steps)
+        if a.out:
+            with open(a.out, "w") as f:
+                json.dump(res, f, indent=1)
+        return
+    if a.cmd == "ensemble":
+        from .ensemble import Ensemble
+
+        ens = Ensemble(((64, 8), (96, 8), (128, 8)), precision="fp32", seed=0)
+        ens.capture(steps_per_graph=1)
+        rate = ens.run(a.steps)
+        res = {"members": [[d.N, d.Cs] for d in ens.members], "steps": a.steps,
+               "grid_points_x_steps_per_s": rate, "dt_last": ens.dt_values()}
+        print(json.dumps({"grid_points_x_steps_per_s": rate, "steps": a.steps}), flush=True)
+        if a.out:
+            with open(a.out, "w") as f:
+                json.dump(res, f, indent=1)
+        return
     if a.cmd == "precision":
         res = precision_study(a.n, tuple(a.steps), les=a.les)
         if a.out:
