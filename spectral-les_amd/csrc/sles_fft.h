@@ -37,6 +37,19 @@ struct alignas(2 * sizeof(T)) cpx {
   T x, y;
 };
 
+// Element idx of an array whose base pointer is the same for the whole wave.
+// The byte offset is formed in 32 bits (every field addressed this way is
+// below 4 GiB), so the access can use the scalar base register plus one
+// 32-bit vector offset instead of a 64-bit address per element.
+template <typename T>
+__device__ __forceinline__ T ld32(const T* base, unsigned idx) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + idx * (unsigned)sizeof(T));
+}
+template <typename T>
+__device__ __forceinline__ void st32(T* base, unsigned idx, T v) {
+  *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + idx * (unsigned)sizeof(T)) = v;
+}
+
 template <typename T>
 __device__ __forceinline__ cpx<T> operator+(cpx<T> a, cpx<T> b) { return {a.x + b.x, a.y + b.y}; }
 template <typename T>
@@ -201,11 +214,81 @@ struct LdsColumn {
 };
 
 // ---------------------------------------------------------------------------
-// One stage: twiddle + butterflies on the P register slots of thread t.
-// tw[e] = exp(-2 pi i e / N) (forward table; the inverse conjugates it).
+// Twiddle sources.  tw[e] = exp(-2 pi i e / N) (forward table; the inverse
+// conjugates it).  Stage (Ns, R) of thread t needs, for each of its Q = P/R
+// butterflies j = t + q*NT and r = 1..R-1, the factor tw[r * (j mod Ns) * N/(Ns R)].
+//   TwTable: read from the table (L1/L2-resident) at every stage.
+//   TwRegs:  every factor of every stage read once into registers (kernels
+//            that run many transforms per thread reuse them).
+// OFF is the running index of the stage's first factor in TwRegs.
 // ---------------------------------------------------------------------------
-template <typename T, int N, int P, int DIR, int R, int Ns>
-__device__ __forceinline__ void line_stage(cpx<T> (&v)[P], int t, const cpx<T>* __restrict__ tw) {
+template <typename T>
+struct TwTable {
+  const cpx<T>* __restrict__ tw;
+  template <int N, int NT, int Ns, int R, int OFF>
+  __device__ __forceinline__ cpx<T> get(int q, int r, int t) const {
+    const int k = (t + q * NT) % Ns;
+    return tw[r * k * (N / (Ns * R))];
+  }
+};
+
+// Number of register twiddles of a plan (stages after the first).
+template <int P, int Ns, int... Rs>
+struct TwCount;
+template <int P, int Ns>
+struct TwCount<P, Ns> {
+  static constexpr int value = 0;
+};
+template <int P, int Ns, int R, int... Rest>
+struct TwCount<P, Ns, R, Rest...> {
+  static constexpr int value = (Ns > 1 ? (P / R) * (R - 1) : 0) + TwCount<P, Ns * R, Rest...>::value;
+};
+
+template <typename T, int N, int P, int Ns, int OFF, int... Rs>
+struct TwLoad;
+template <typename T, int N, int P, int Ns, int OFF>
+struct TwLoad<T, N, P, Ns, OFF> {
+  __device__ __forceinline__ static void run(cpx<T>*, const cpx<T>* __restrict__, int) {}
+};
+template <typename T, int N, int P, int Ns, int OFF, int R, int... Rest>
+struct TwLoad<T, N, P, Ns, OFF, R, Rest...> {
+  static constexpr int CNT = Ns > 1 ? (P / R) * (R - 1) : 0;
+  __device__ __forceinline__ static void run(cpx<T>* w, const cpx<T>* __restrict__ tw, int t) {
+    if constexpr (Ns > 1) {
+      constexpr int NT = N / P, Q = P / R;
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+        const int k = (t + q * NT) % Ns;
+#pragma unroll
+        for (int r = 1; r < R; ++r) w[OFF + q * (R - 1) + (r - 1)] = tw[r * k * (N / (Ns * R))];
+      }
+    }
+    TwLoad<T, N, P, Ns * R, OFF + CNT, Rest...>::run(w, tw, t);
+  }
+};
+
+template <typename T, class PL>
+struct TwRegs {
+  template <int... Rs>
+  using count_of = TwCount<PL::P, 1, Rs...>;
+  static constexpr int COUNT = PL::template apply<count_of>::value;
+  template <int... Rs>
+  using loader = TwLoad<T, PL::N, PL::P, 1, 0, Rs...>;
+  cpx<T> w[COUNT > 0 ? COUNT : 1];
+  __device__ __forceinline__ TwRegs(const cpx<T>* __restrict__ tw, int t) {
+    PL::template apply<loader>::run(w, tw, t);
+  }
+  template <int N, int NT, int Ns, int R, int OFF>
+  __device__ __forceinline__ cpx<T> get(int q, int r, int) const {
+    return w[OFF + q * (R - 1) + (r - 1)];
+  }
+};
+
+// ---------------------------------------------------------------------------
+// One stage: twiddle + butterflies on the P register slots of thread t.
+// ---------------------------------------------------------------------------
+template <typename T, int N, int P, int DIR, int R, int Ns, int OFF, class TW>
+__device__ __forceinline__ void line_stage(cpx<T> (&v)[P], int t, const TW& tw) {
   constexpr int NT = N / P;
   constexpr int Q = P / R;
   static_assert(P % R == 0, "radix must divide points per thread");
@@ -215,12 +298,9 @@ __device__ __forceinline__ void line_stage(cpx<T> (&v)[P], int t, const cpx<T>* 
 #pragma unroll
     for (int r = 0; r < R; ++r) a[r] = v[q + r * Q];
     if constexpr (Ns > 1) {
-      const int j = t + q * NT;
-      const int k = j % Ns;
-      constexpr int step = N / (Ns * R);
 #pragma unroll
       for (int r = 1; r < R; ++r) {
-        cpx<T> w = tw[r * k * step];
+        cpx<T> w = tw.template get<N, NT, Ns, R, OFF>(q, r, t);
         if constexpr (DIR > 0) w.y = -w.y;
         a[r] = cmul(a[r], w);
       }
@@ -249,26 +329,25 @@ __device__ __forceinline__ void line_exchange(cpx<T> (&v)[P], cpx<T>* lds, const
   block_sync();
 }
 
-template <typename T, int N, int P, int DIR, int Ns, int... Rs>
+template <typename T, int N, int P, int DIR, int Ns, int OFF, int... Rs>
 struct LineStages;
 
-template <typename T, int N, int P, int DIR, int Ns, int R>
-struct LineStages<T, N, P, DIR, Ns, R> {
-  template <class Map>
-  __device__ __forceinline__ static void run(cpx<T> (&v)[P], cpx<T>* lds, const Map& map, int t,
-                                             const cpx<T>* __restrict__ tw) {
-    line_stage<T, N, P, DIR, R, Ns>(v, t, tw);
+template <typename T, int N, int P, int DIR, int Ns, int OFF, int R>
+struct LineStages<T, N, P, DIR, Ns, OFF, R> {
+  template <class Map, class TW>
+  __device__ __forceinline__ static void run(cpx<T> (&v)[P], cpx<T>* lds, const Map& map, int t, const TW& tw) {
+    line_stage<T, N, P, DIR, R, Ns, OFF>(v, t, tw);
   }
 };
 
-template <typename T, int N, int P, int DIR, int Ns, int R, int R2, int... Rest>
-struct LineStages<T, N, P, DIR, Ns, R, R2, Rest...> {
-  template <class Map>
-  __device__ __forceinline__ static void run(cpx<T> (&v)[P], cpx<T>* lds, const Map& map, int t,
-                                             const cpx<T>* __restrict__ tw) {
-    line_stage<T, N, P, DIR, R, Ns>(v, t, tw);
+template <typename T, int N, int P, int DIR, int Ns, int OFF, int R, int R2, int... Rest>
+struct LineStages<T, N, P, DIR, Ns, OFF, R, R2, Rest...> {
+  static constexpr int NEXT = OFF + (Ns > 1 ? (P / R) * (R - 1) : 0);
+  template <class Map, class TW>
+  __device__ __forceinline__ static void run(cpx<T> (&v)[P], cpx<T>* lds, const Map& map, int t, const TW& tw) {
+    line_stage<T, N, P, DIR, R, Ns, OFF>(v, t, tw);
     line_exchange<T, N, P, R, Ns>(v, lds, map, t);
-    LineStages<T, N, P, DIR, Ns * R, R2, Rest...>::run(v, lds, map, t, tw);
+    LineStages<T, N, P, DIR, Ns * R, NEXT, R2, Rest...>::run(v, lds, map, t, tw);
   }
 };
 
@@ -278,14 +357,19 @@ struct LineFFTP {
   static constexpr int P = PL::P;
   static constexpr int NT = PL::NT;
   template <int... Rs>
-  using stages = LineStages<T, N, P, DIR, 1, Rs...>;
+  using stages = LineStages<T, N, P, DIR, 1, 0, Rs...>;
   using S = typename PL::template apply<stages>;
   // Transform the line held by thread t (0 <= t < NT).  Every thread of the
   // block must call this (it contains block barriers when the plan has more
-  // than one stage).
+  // than one stage).  tw: the twiddle table, or a TwRegs<T, PL> of thread t.
   template <class Map>
   __device__ __forceinline__ static void run(cpx<T> (&v)[P], cpx<T>* lds, const Map& map, int t,
                                              const cpx<T>* __restrict__ tw) {
+    S::run(v, lds, map, t, TwTable<T>{tw});
+  }
+  template <class Map>
+  __device__ __forceinline__ static void run(cpx<T> (&v)[P], cpx<T>* lds, const Map& map, int t,
+                                             const TwRegs<T, PL>& tw) {
     S::run(v, lds, map, t, tw);
   }
 };

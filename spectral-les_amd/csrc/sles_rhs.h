@@ -38,6 +38,14 @@ namespace sles {
 
 enum { EPI_RK4 = 0, EPI_RK3 = 1, EPI_ETD2 = 2, EPI_RHS = 3, EPI_NONLIN = 4 };
 
+// Element index inside one spectral field and the accessors of the strided
+// passes.
+using eidx = long;
+template <typename T>
+__device__ __forceinline__ T ldx(const T* b, eidx i) { return b[i]; }
+template <typename T>
+__device__ __forceinline__ void stx(T* b, eidx i, T v) { b[i] = v; }
+
 struct StepArgs {
   double dt;             // host step size (when dt_dev == nullptr)
   const double* dt_dev;  // on-device step size, or nullptr
@@ -86,10 +94,11 @@ __global__ void __launch_bounds__((StridedShape<T, N, PL>::THREADS))
   __shared__ cpx<T> lds[N * TK];
   const Cols<N, TK> col(xcd);
   const int t = threadIdx.x / TK;
+  const TwRegs<T, PL> twr(tw, t);  // read once per block, used by every transform
   const LdsColumn<TK> map{col.c};
   const T ky = (T)kfreq((int)(col.g / Cols<N, TK>::NH), N);
   const T kz = (T)(col.g % Cols<N, TK>::NH);
-  auto at = [&](int m) { return (long)(t + m * NT) * Cols<N, TK>::M + col.g; };
+  auto at = [&](int m) { return (eidx)(t + m * NT) * (eidx)Cols<N, TK>::M + (eidx)col.g; };
   cpx<T> v[P];
   if constexpr (LES) {
 #pragma unroll 1
@@ -97,27 +106,27 @@ __global__ void __launch_bounds__((StridedShape<T, N, PL>::THREADS))
       cpx<T> w[P];
 #pragma unroll
       for (int m = 0; m < P; ++m) {
-        v[m] = uh[j * C + at(m)];
+        v[m] = ldx(uh + j * C, at(m));
         w[m] = mul_i((T)kfreq(t + m * NT, N) * v[m]);  // 1j * K0 * U_j
       }
-      FFT::run(v, lds, map, t, tw);
+      FFT::run(v, lds, map, t, twr);
       if (col.valid)
 #pragma unroll
-        for (int m = 0; m < P; ++m) W[j * C + at(m)] = v[m];
-      FFT::run(w, lds, map, t, tw);
+        for (int m = 0; m < P; ++m) stx(W + j * C, at(m), v[m]);
+      FFT::run(w, lds, map, t, twr);
       if (col.valid)
 #pragma unroll
-        for (int m = 0; m < P; ++m) W[(3 + j) * C + at(m)] = w[m];
+        for (int m = 0; m < P; ++m) stx(W + (3 + j) * C, at(m), w[m]);
     }
   } else {
 #pragma unroll 1
     for (int j = 0; j < 3; ++j) {
 #pragma unroll
-      for (int m = 0; m < P; ++m) v[m] = uh[j * C + at(m)];
-      FFT::run(v, lds, map, t, tw);
+      for (int m = 0; m < P; ++m) v[m] = ldx(uh + j * C, at(m));
+      FFT::run(v, lds, map, t, twr);
       if (col.valid)
 #pragma unroll
-        for (int m = 0; m < P; ++m) W[j * C + at(m)] = v[m];
+        for (int m = 0; m < P; ++m) stx(W + j * C, at(m), v[m]);
     }
     // omega_hat = 1j * (K x U_hat)  (math_formula.py:22-24)
 #pragma unroll 1
@@ -125,22 +134,22 @@ __global__ void __launch_bounds__((StridedShape<T, N, PL>::THREADS))
 #pragma unroll
       for (int m = 0; m < P; ++m) {
         const T kx = (T)kfreq(t + m * NT, N);
-        const long e = at(m);
+        const eidx e = at(m);
         if (j == 0) {
-          const cpx<T> u1 = uh[C + e], u2 = uh[2 * C + e];
+          const cpx<T> u1 = ldx(uh + C, e), u2 = ldx(uh + 2 * C, e);
           v[m] = mul_i(ky * u2 - kz * u1);
         } else if (j == 1) {
-          const cpx<T> u0 = uh[e], u2 = uh[2 * C + e];
+          const cpx<T> u0 = ldx(uh, e), u2 = ldx(uh + 2 * C, e);
           v[m] = mul_i(kz * u0 - kx * u2);
         } else {
-          const cpx<T> u0 = uh[e], u1 = uh[C + e];
+          const cpx<T> u0 = ldx(uh, e), u1 = ldx(uh + C, e);
           v[m] = mul_i(kx * u1 - ky * u0);
         }
       }
-      FFT::run(v, lds, map, t, tw);
+      FFT::run(v, lds, map, t, twr);
       if (col.valid)
 #pragma unroll
-        for (int m = 0; m < P; ++m) W[(3 + j) * C + at(m)] = v[m];
+        for (int m = 0; m < P; ++m) stx(W + (3 + j) * C, at(m), v[m]);
     }
   }
 }
@@ -157,9 +166,10 @@ __global__ void __launch_bounds__((StridedShape<T, N, PL>::THREADS))
   __shared__ cpx<T> lds[N * TK];
   const Cols<N, TK> col(xcd);
   const int t = threadIdx.x / TK;
+  const TwRegs<T, PL> twr(tw, t);  // read once per block, used by every transform
   const LdsColumn<TK> map{col.c};
   const long ix = col.g / NH, iz = col.g % NH;
-  auto at = [&](int m) { return ix * (long)N * NH + (long)(t + m * NT) * NH + iz; };
+  auto at = [&](int m) { return (eidx)ix * (eidx)(N * NH) + (eidx)(t + m * NT) * NH + (eidx)iz; };
   cpx<T> v[P];
   if constexpr (LES) {
 #pragma unroll 1
@@ -167,27 +177,27 @@ __global__ void __launch_bounds__((StridedShape<T, N, PL>::THREADS))
       cpx<T> w[P];
 #pragma unroll
       for (int m = 0; m < P; ++m) {
-        v[m] = W[j * C + at(m)];
+        v[m] = ldx(W + j * C, at(m));
         w[m] = mul_i((T)kfreq(t + m * NT, N) * v[m]);  // 1j * K1 * (.)
       }
-      FFT::run(v, lds, map, t, tw);
+      FFT::run(v, lds, map, t, twr);
       if (col.valid)
 #pragma unroll
-        for (int m = 0; m < P; ++m) W[j * C + at(m)] = v[m];
-      FFT::run(w, lds, map, t, tw);
+        for (int m = 0; m < P; ++m) stx(W + j * C, at(m), v[m]);
+      FFT::run(w, lds, map, t, twr);
       if (col.valid)
 #pragma unroll
-        for (int m = 0; m < P; ++m) W[(6 + j) * C + at(m)] = w[m];
+        for (int m = 0; m < P; ++m) stx(W + (6 + j) * C, at(m), w[m]);
     }
   }
 #pragma unroll 1
   for (int j = LES ? 3 : 0; j < 6; ++j) {
 #pragma unroll
-    for (int m = 0; m < P; ++m) v[m] = W[j * C + at(m)];
-    FFT::run(v, lds, map, t, tw);
+    for (int m = 0; m < P; ++m) v[m] = ldx(W + j * C, at(m));
+    FFT::run(v, lds, map, t, twr);
     if (col.valid)
 #pragma unroll
-      for (int m = 0; m < P; ++m) W[j * C + at(m)] = v[m];
+      for (int m = 0; m < P; ++m) stx(W + j * C, at(m), v[m]);
   }
 }
 
@@ -213,6 +223,28 @@ __device__ __forceinline__ void load_pair(cpx<T> (&v)[P], const cpx<T>* A, bool 
     // A, B point at an in-bounds row even for an invalid line (never stored)
     const int kk = k <= N / 2 ? k : N - k;
     cpx<T> a = A[kk], b = B[kk];
+    if (dzA) a = mul_i((T)kk * a);
+    if (dzB) b = mul_i((T)kk * b);
+    if (k > N / 2) {
+      a = conj(a);
+      b = conj(b);
+    } else if (k == 0 || k == N / 2) {
+      a.y = T(0);
+      b.y = T(0);
+    }
+    v[m] = {a.x - b.y, a.y + b.x};
+  }
+}
+
+// Same with uniform field bases A, B and the row's first element `row0`.
+template <typename T, int N, int P, int NT>
+__device__ __forceinline__ void load_pair32(cpx<T> (&v)[P], const cpx<T>* A, bool dzA, const cpx<T>* B, bool dzB,
+                                            unsigned row0, int t) {
+#pragma unroll
+  for (int m = 0; m < P; ++m) {
+    const int k = t + m * NT;
+    const int kk = k <= N / 2 ? k : N - k;
+    cpx<T> a = ld32(A, row0 + kk), b = ld32(B, row0 + kk);
     if (dzA) a = mul_i((T)kk * a);
     if (dzB) b = mul_i((T)kk * b);
     if (k > N / 2) {
@@ -280,18 +312,19 @@ __global__ void __launch_bounds__(ZLines<PL>::THREADS)
   const int line = threadIdx.x / NT, t = threadIdx.x % NT;
   const long row = (long)blockIdx.x * LPB + line;
   const bool valid = row < (long)N * N;
-  const long base = (valid ? row : 0) * NH;
+  const unsigned row0 = (unsigned)((valid ? row : 0) * NH);
   const LdsLine<T> map{line * LdsLine<T>::stride(N)};
-  const cpx<T>* w[9];
+  const TwRegs<T, PL> twr(tw, t);  // read once, used by all 11 transforms
+  cpx<T>* w[9];
 #pragma unroll
-  for (int i = 0; i < 9; ++i) w[i] = W + i * C + base;
+  for (int i = 0; i < 9; ++i) w[i] = W + i * C;
 
   // ---- inverse: 6 complex transforms -> 12 physical fields (scaled 1/N^3)
   T u0[P], u1[P], u2[P], a11[P], a12[P], a13[P], a21[P], a22[P], a23[P], a31[P], a32[P], a33[P];
   cpx<T> v[P];
 #define SLES_INV(A, DA, B, DB, RA, RB)                             \
-  load_pair<T, N, P, NT>(v, A, DA, B, DB, t, valid);               \
-  IFFT::run(v, lds, map, t, tw);                                   \
+  load_pair32<T, N, P, NT>(v, A, DA, B, DB, row0, t);              \
+  IFFT::run(v, lds, map, t, twr);                                  \
   _Pragma("unroll") for (int m = 0; m < P; ++m) {                  \
     RA[m] = scale * v[m].x;                                        \
     RB[m] = scale * v[m].y;                                        \
@@ -343,21 +376,18 @@ __global__ void __launch_bounds__(ZLines<PL>::THREADS)
 
   // ---- forward: 5 complex transforms -> 8 output half-spectra
   cpx<T> A[KH], B[KH], t13[KH], t23[KH];
-  cpx<T>* o[9];
-#pragma unroll
-  for (int i = 0; i < 9; ++i) o[i] = W + i * C + base;
   auto kok = [&](int m) { return t + m * NT <= N / 2; };
 #define SLES_FWD(QA, QB)                                     \
   _Pragma("unroll") for (int m = 0; m < P; ++m) v[m] = {QA[m], QB[m]}; \
-  FFT::run(v, lds, map, t, tw);                              \
+  FFT::run(v, lds, map, t, twr);                             \
   split_pair<T, N, P, NT>(v, lds, map, t, A, B);
   SLES_FWD(q4, q5)  // tau_xy, tau_xz
 #pragma unroll
   for (int m = 0; m < KH; ++m) {
     t13[m] = B[m];
     if (valid && kok(m)) {
-      o[4][t + m * NT] = A[m];
-      o[5][t + m * NT] = B[m];
+      st32(w[4], row0 + t + m * NT, A[m]);
+      st32(w[5], row0 + t + m * NT, B[m]);
     }
   }
   SLES_FWD(q6, q7)  // tau_yy, tau_yz
@@ -365,23 +395,23 @@ __global__ void __launch_bounds__(ZLines<PL>::THREADS)
   for (int m = 0; m < KH; ++m) {
     t23[m] = B[m];
     if (valid && kok(m)) {
-      o[6][t + m * NT] = A[m];
-      o[7][t + m * NT] = B[m];
+      st32(w[6], row0 + t + m * NT, A[m]);
+      st32(w[7], row0 + t + m * NT, B[m]);
     }
   }
   SLES_FWD(q2, q8)  // N_z, tau_zz
 #pragma unroll
   for (int m = 0; m < KH; ++m) {
     const T kz2 = T(2) * (T)(t + m * NT);
-    if (valid && kok(m)) o[2][t + m * NT] = A[m] + mul_i(kz2 * B[m]);
+    if (valid && kok(m)) st32(w[2], row0 + t + m * NT, A[m] + mul_i(kz2 * B[m]));
   }
   SLES_FWD(q0, q1)  // N_x, N_y
 #pragma unroll
   for (int m = 0; m < KH; ++m) {
     const T kz2 = T(2) * (T)(t + m * NT);
     if (valid && kok(m)) {
-      o[0][t + m * NT] = A[m] + mul_i(kz2 * t13[m]);
-      o[1][t + m * NT] = B[m] + mul_i(kz2 * t23[m]);
+      st32(w[0], row0 + t + m * NT, A[m] + mul_i(kz2 * t13[m]));
+      st32(w[1], row0 + t + m * NT, B[m] + mul_i(kz2 * t23[m]));
     }
   }
   T zq[P];
@@ -390,7 +420,7 @@ __global__ void __launch_bounds__(ZLines<PL>::THREADS)
   SLES_FWD(q3, zq)  // tau_xx
 #pragma unroll
   for (int m = 0; m < KH; ++m)
-    if (valid && kok(m)) o[3][t + m * NT] = A[m];
+    if (valid && kok(m)) st32(w[3], row0 + t + m * NT, A[m]);
 #undef SLES_FWD
 }
 
@@ -405,16 +435,17 @@ __global__ void __launch_bounds__(ZLines<PL>::THREADS)
   const int line = threadIdx.x / NT, t = threadIdx.x % NT;
   const long row = (long)blockIdx.x * LPB + line;
   const bool valid = row < (long)N * N;
-  const long base = (valid ? row : 0) * NH;
+  const unsigned row0 = (unsigned)((valid ? row : 0) * NH);
   const LdsLine<T> map{line * LdsLine<T>::stride(N)};
+  const TwRegs<T, PL> twr(tw, t);  // read once, used by all 5 transforms
   cpx<T>* w[6];
 #pragma unroll
-  for (int i = 0; i < 6; ++i) w[i] = W + i * C + base;
+  for (int i = 0; i < 6; ++i) w[i] = W + i * C;
   T u0[P], u1[P], u2[P], o0[P], o1[P], o2[P];
   cpx<T> v[P];
 #define SLES_INV(A, B, RA, RB)                                     \
-  load_pair<T, N, P, NT>(v, A, false, B, false, t, valid);         \
-  IFFT::run(v, lds, map, t, tw);                                   \
+  load_pair32<T, N, P, NT>(v, A, false, B, false, row0, t);       \
+  IFFT::run(v, lds, map, t, twr);                                  \
   _Pragma("unroll") for (int m = 0; m < P; ++m) {                  \
     RA[m] = scale * v[m].x;                                        \
     RB[m] = scale * v[m].y;                                        \
@@ -436,21 +467,21 @@ __global__ void __launch_bounds__(ZLines<PL>::THREADS)
   cpx<T> A[KH], B[KH];
 #pragma unroll
   for (int m = 0; m < P; ++m) v[m] = {n0[m], n1[m]};
-  FFT::run(v, lds, map, t, tw);
+  FFT::run(v, lds, map, t, twr);
   split_pair<T, N, P, NT>(v, lds, map, t, A, B);
 #pragma unroll
   for (int m = 0; m < KH; ++m)
     if (valid && t + m * NT <= N / 2) {
-      w[0][t + m * NT] = A[m];
-      w[1][t + m * NT] = B[m];
+      st32(w[0], row0 + t + m * NT, A[m]);
+      st32(w[1], row0 + t + m * NT, B[m]);
     }
 #pragma unroll
   for (int m = 0; m < P; ++m) v[m] = {n2[m], T(0)};
-  FFT::run(v, lds, map, t, tw);
+  FFT::run(v, lds, map, t, twr);
   split_pair<T, N, P, NT>(v, lds, map, t, A, B);
 #pragma unroll
   for (int m = 0; m < KH; ++m)
-    if (valid && t + m * NT <= N / 2) w[2][t + m * NT] = A[m];
+    if (valid && t + m * NT <= N / 2) st32(w[2], row0 + t + m * NT, A[m]);
 }
 
 // ---------------------------------------------------------------------------
@@ -465,30 +496,31 @@ __global__ void __launch_bounds__((StridedShape<T, N, PL>::THREADS))
   __shared__ cpx<T> lds[N * TK];
   const Cols<N, TK> col(xcd);
   const int t = threadIdx.x / TK;
+  const TwRegs<T, PL> twr(tw, t);  // read once per block, used by every transform
   const LdsColumn<TK> map{col.c};
   const long ix = col.g / NH, iz = col.g % NH;
-  auto at = [&](int m) { return ix * (long)N * NH + (long)(t + m * NT) * NH + iz; };
+  auto at = [&](int m) { return (eidx)ix * (eidx)(N * NH) + (eidx)(t + m * NT) * NH + (eidx)iz; };
   cpx<T> v[P], acc[P];
   auto load = [&](int f) {
 #pragma unroll
-    for (int m = 0; m < P; ++m) v[m] = W[f * C + at(m)];
+    for (int m = 0; m < P; ++m) v[m] = ldx(W + f * C, at(m));
   };
   auto store = [&](int f, const cpx<T>(&x)[P]) {
     if (col.valid)
 #pragma unroll
-      for (int m = 0; m < P; ++m) W[f * C + at(m)] = x[m];
+      for (int m = 0; m < P; ++m) stx(W + f * C, at(m), x[m]);
   };
   if constexpr (LES) {
     // H_i = F_y(G_i) + 2i Ky F_y(tau_iy): tau_xy (W4), tau_yy (W6), tau_yz (W7)
 #pragma unroll 1
     for (int i = 0; i < 3; ++i) {
       load(i == 0 ? 4 : (i == 1 ? 6 : 7));
-      FFT::run(v, lds, map, t, tw);
+      FFT::run(v, lds, map, t, twr);
       if (i == 0) store(4, v);
 #pragma unroll
       for (int m = 0; m < P; ++m) acc[m] = mul_i((T(2) * (T)kfreq(t + m * NT, N)) * v[m]);
       load(i);
-      FFT::run(v, lds, map, t, tw);
+      FFT::run(v, lds, map, t, twr);
 #pragma unroll
       for (int m = 0; m < P; ++m) acc[m] = v[m] + acc[m];
       store(i, acc);
@@ -496,14 +528,14 @@ __global__ void __launch_bounds__((StridedShape<T, N, PL>::THREADS))
 #pragma unroll 1
     for (int r = 3; r <= 5; r += 2) {
       load(r);
-      FFT::run(v, lds, map, t, tw);
+      FFT::run(v, lds, map, t, twr);
       store(r, v);
     }
   } else {
 #pragma unroll 1
     for (int f = 0; f < 3; ++f) {
       load(f);
-      FFT::run(v, lds, map, t, tw);
+      FFT::run(v, lds, map, t, twr);
       store(f, v);
     }
   }
@@ -522,7 +554,7 @@ __device__ __forceinline__ double phi2_d(double z) {
 }
 
 template <typename T>
-__device__ __forceinline__ void epilogue(cpx<T> (&s)[3], long e, int kx, int ky, int kz, int n, const StepArgs& a,
+__device__ __forceinline__ void epilogue(cpx<T> (&s)[3], eidx e, int kx, int ky, int kz, int n, const StepArgs& a,
                                          double dt, cpx<T>* uh, cpx<T>* uh0, cpx<T>* uh1, cpx<T>* aux, long C) {
   const KIndex k = {kx, ky, kz};
   const T keep = dealias_keep(k, n) ? T(1) : T(0);
@@ -539,13 +571,13 @@ __device__ __forceinline__ void epilogue(cpx<T> (&s)[3], long e, int kx, int ky,
   cpx<T> u[3];
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
-    u[i] = uh[i * C + e];
+    u[i] = ldx(uh + i * C, e);
     d[i] = d[i] - kk[i] * p;
     if (a.forcing > 0.0 && k2 != 0) d[i] = d[i] + (T)a.forcing * u[i];
   }
   if (a.scheme == EPI_NONLIN) {
 #pragma unroll
-    for (int i = 0; i < 3; ++i) aux[i * C + e] = d[i];
+    for (int i = 0; i < 3; ++i) stx(aux + i * C, e, d[i]);
     return;
   }
   if (a.scheme == EPI_ETD2) {
@@ -555,15 +587,15 @@ __device__ __forceinline__ void epilogue(cpx<T> (&s)[3], long e, int kx, int ky,
     const T ez = (T)exp(z), w1 = (T)(dt * (p1 + p2)), w0 = (T)(dt * p2);
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
-      const cpx<T> f0 = aux[i * C + e];
-      uh[i * C + e] = (ez * u[i] + w1 * d[i]) - w0 * f0;
-      aux[i * C + e] = d[i];
+      const cpx<T> f0 = ldx(aux + i * C, e);
+      stx(uh + i * C, e, (ez * u[i] + w1 * d[i]) - w0 * f0);
+      stx(aux + i * C, e, d[i]);
     }
     return;
   }
   if (a.scheme == EPI_RK4 && a.stage == 0 && aux) {
 #pragma unroll
-    for (int i = 0; i < 3; ++i) aux[i * C + e] = d[i];  // F(u_0), kept for an ETD2 start
+    for (int i = 0; i < 3; ++i) stx(aux + i * C, e, d[i]);  // F(u_0), kept for an ETD2 start
   }
   if (a.nu > 0.0) {
     const T c = (T)(a.nu * (double)k2);
@@ -572,25 +604,27 @@ __device__ __forceinline__ void epilogue(cpx<T> (&s)[3], long e, int kx, int ky,
   }
   if (a.scheme == EPI_RHS) {
 #pragma unroll
-    for (int i = 0; i < 3; ++i) aux[i * C + e] = d[i];
+    for (int i = 0; i < 3; ++i) stx(aux + i * C, e, d[i]);
     return;
   }
   if (a.scheme == EPI_RK4) {
     const T ca = (T)(a.ca * dt), cb = (T)(a.cb * dt);
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
-      const long ei = i * C + e;
+      cpx<T>* const ui = uh + i * C;
+      cpx<T>* const u0i = uh0 + i * C;
+      cpx<T>* const u1i = uh1 + i * C;
       if (a.stage == 0) {
-        uh0[ei] = u[i];
-        uh1[ei] = u[i] + ca * d[i];
-        uh[ei] = u[i] + cb * d[i];
+        stx(u0i, e, u[i]);
+        stx(u1i, e, u[i] + ca * d[i]);
+        stx(ui, e, u[i] + cb * d[i]);
       } else if (a.stage < 3) {
-        uh1[ei] = uh1[ei] + ca * d[i];
-        uh[ei] = uh0[ei] + cb * d[i];
+        stx(u1i, e, ldx(u1i, e) + ca * d[i]);
+        stx(ui, e, ldx(u0i, e) + cb * d[i]);
       } else {
-        const cpx<T> fin = uh1[ei] + ca * d[i];
-        uh1[ei] = fin;
-        uh[ei] = fin;
+        const cpx<T> fin = ldx(u1i, e) + ca * d[i];
+        stx(u1i, e, fin);
+        stx(ui, e, fin);
       }
     }
     return;
@@ -599,24 +633,28 @@ __device__ __forceinline__ void epilogue(cpx<T> (&s)[3], long e, int kx, int ky,
   const T hdt = (T)(0.5 * dt), fdt = (T)dt, tdt = (T)(2.0 * dt);
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
-    const long ei = i * C + e;
+    cpx<T>* const ui = uh + i * C;
+    cpx<T>* const u0i = uh0 + i * C;
     if (a.stage == 0) {
-      uh0[ei] = u[i];
-      uh[ei] = u[i] + hdt * d[i];
+      stx(u0i, e, u[i]);
+      stx(ui, e, u[i] + hdt * d[i]);
     } else if (a.stage == 1) {
-      uh[ei] = (uh0[ei] - fdt * d[i]) + tdt * d[i];
+      stx(ui, e, (ldx(u0i, e) - fdt * d[i]) + tdt * d[i]);
     } else {
       const cpx<T> sm = ((T(1) / T(6)) * d[i] + (T(2) / T(3)) * d[i]) + (T(1) / T(6)) * d[i];
-      uh[ei] = uh0[ei] + fdt * sm;
+      stx(ui, e, ldx(u0i, e) + fdt * sm);
     }
   }
 }
 
+// Blocks of at most 512 threads for every plan: the three accumulated
+// components, the transform and the epilogue need ~180 VGPRs per thread with
+// the radix-4 FP64 plan, which a 1024-thread block (128 VGPRs) would spill.
 template <typename T, int N, int LES, class PL>
-__global__ void __launch_bounds__((StridedShape<T, N, PL>::THREADS))
+__global__ void __launch_bounds__((StridedShape<T, N, PL, 512>::THREADS))
     k_xfwd(cpx<T>* __restrict__ W, long C, cpx<T>* uh, cpx<T>* uh0, cpx<T>* uh1, cpx<T>* aux, StepArgs a,
            const cpx<T>* __restrict__ tw, int xcd) {
-  using S = StridedShape<T, N, PL>;
+  using S = StridedShape<T, N, PL, 512>;
   constexpr int P = S::P, NT = S::NT, TK = S::TK, NH = N / 2 + 1;
   using FFT = LineFFTP<T, PL, -1>;
   __shared__ cpx<T> lds[N * TK];
@@ -627,16 +665,17 @@ __global__ void __launch_bounds__((StridedShape<T, N, PL>::THREADS))
   }
   const Cols<N, TK> col(xcd);
   const int t = threadIdx.x / TK;
+  const TwRegs<T, PL> twr(tw, t);  // read once per block, used by every transform
   const LdsColumn<TK> map{col.c};
-  auto at = [&](int m) { return (long)(t + m * NT) * Cols<N, TK>::M + col.g; };
+  auto at = [&](int m) { return (eidx)(t + m * NT) * (eidx)Cols<N, TK>::M + (eidx)col.g; };
   // a0..a2: S_i = F_x(W_i) (+ 2i Kx F_x(tau_xi) with LES).  Named arrays and
   // a rolled field loop with uniform branches keep them in registers.
   cpx<T> a0[P], a1[P], a2[P], v[P];
 #pragma unroll 1
   for (int f = 0; f < (LES ? 6 : 3); ++f) {
 #pragma unroll
-    for (int m = 0; m < P; ++m) v[m] = W[f * C + at(m)];
-    FFT::run(v, lds, map, t, tw);
+    for (int m = 0; m < P; ++m) v[m] = ldx(W + f * C, at(m));
+    FFT::run(v, lds, map, t, twr);
     if (f < 3) {
       if (f == 0) {
 #pragma unroll
@@ -740,7 +779,7 @@ struct ProfScope {
 template <typename T, int N, class PS, class PX, class PZ>
 struct Fused {
   using SS = StridedShape<T, N, PS>;  // x/y inverse and y forward
-  using SX = StridedShape<T, N, PX>;  // x forward + stage update
+  using SX = StridedShape<T, N, PX, 512>;  // x forward + stage update
   static constexpr int ZB = (int)(((long)N * N + ZLines<PZ>::LPB - 1) / ZLines<PZ>::LPB);
 
   static int stage(const sles_plan_s* p, int les, T coeff, cpx<T>* uh, cpx<T>* W, double* part, hipStream_t s) {

@@ -38,9 +38,11 @@ def isotropic_spectrum(N, k0=4.0, seed=0, urms=1.0, device="cuda", dtype=None):
         and normalised,
       * k = 0, Nyquist rows (|kx| or |ky| or kz = N/2) and dealiased modes
         zeroed,
-      * Hermitian symmetry of the kz = 0 plane enforced by a c2r / r2c round
-        trip on the GPU (so the physical field is exactly the one stored),
-      * rescaled to u_rms = sqrt(<|u|^2>/3) = urms.
+      * Hermitian symmetry of the kz = 0 plane enforced by averaging
+        U(kx, ky, 0) with conj U(-kx, -ky, 0) (so the spectrum is the
+        transform of a real field and the zeroed modes stay exactly zero),
+      * rescaled to u_rms = sqrt(<|u|^2>/3) = urms (sum of squares of the
+        GPU inverse transform, FP64).
     """
     import numpy as np
 
@@ -77,6 +79,11 @@ def isotropic_spectrum(N, k0=4.0, seed=0, urms=1.0, device="cuda", dtype=None):
     del amp, norm, keep, kmag, kk2
     xi *= scale[None]
     del scale
+    # kz = 0 plane: index -k of fftfreq order is (N - i) % N
+    neg = (-np.arange(N)) % N
+    plane = xi[:, :, :, 0]
+    xi[:, :, :, 0] = 0.5 * (plane + np.conj(plane[:, neg][:, :, neg]))
+    del plane
     U_hat = torch.as_tensor(xi, device=device).to(dtype)
     del xi
     plan = _lib.Plan(N, _lib.SLES_FP64 if rdtype == torch.float64 else _lib.SLES_FP32, device)
@@ -88,7 +95,7 @@ def isotropic_spectrum(N, k0=4.0, seed=0, urms=1.0, device="cuda", dtype=None):
     _lib.call("sles_sum_squares", rplan.handle, _lib.ptr_array([u[0], u[1], u[2]]), 3, rout.data_ptr(),
               rwork.data_ptr(), _lib.stream_handle(torch.device(device)))
     ms = float(rout.item()) / (3.0 * N ** 3)  # <|u|^2>/3
-    fft.forward(u, U_hat, normalize=False)
+    del u
     _lib.call("sles_scale", plan.handle, U_hat.data_ptr(), 3, urms / np.sqrt(ms),
               _lib.stream_handle(torch.device(device)))
     torch.cuda.synchronize(device)
