@@ -67,6 +67,7 @@ __global__ void __launch_bounds__((StridedShape<T, N, PL>::THREADS))
   __shared__ cpx<T> lds[N * TK];
   const Cols<N, TK> col(xcd);
   const int t = threadIdx.x / TK;
+  const TwRegs<T, PL> twr(tw, t);  // read once per block, used by every transform
   const LdsColumn<TK> map{col.c};
   const int ky = kfreq((int)(col.g / NH), N), kz = (int)(col.g % NH);
   auto at = [&](int m) { return (long)(t + m * NT) * Cols<N, TK>::M + col.g; };
@@ -79,10 +80,10 @@ __global__ void __launch_bounds__((StridedShape<T, N, PL>::THREADS))
       v[m] = Y[j * C + at(m)];
       w[m] = mul_i((T)kfreq(t + m * NT, N) * v[m]);
     }
-    FFT::run(v, lds, map, t, tw);
+    FFT::run(v, lds, map, t, twr);
 #pragma unroll
     for (int m = 0; m < P; ++m) A[j * C + at(m)] = v[m];
-    FFT::run(w, lds, map, t, tw);
+    FFT::run(w, lds, map, t, twr);
 #pragma unroll
     for (int m = 0; m < P; ++m) A[(3 + j) * C + at(m)] = w[m];
   }
@@ -108,10 +109,10 @@ __global__ void __launch_bounds__((StridedShape<T, N, PL>::THREADS))
       v[m] = keep * (mu[j * C + at(m)] - kj * s[m]);
       w[m] = mul_mis(T(2) * (T)kx, v[m]);
     }
-    FFT::run(v, lds, map, t, tw);
+    FFT::run(v, lds, map, t, twr);
 #pragma unroll
     for (int m = 0; m < P; ++m) A[(9 + j) * C + at(m)] = v[m];
-    FFT::run(w, lds, map, t, tw);
+    FFT::run(w, lds, map, t, twr);
 #pragma unroll
     for (int m = 0; m < P; ++m) A[(12 + j) * C + at(m)] = w[m];
   }
@@ -129,6 +130,7 @@ __global__ void __launch_bounds__((StridedShape<T, N, PL>::THREADS))
   __shared__ cpx<T> lds[N * TK];
   const Cols<N, TK> col(xcd);
   const int t = threadIdx.x / TK;
+  const TwRegs<T, PL> twr(tw, t);  // read once per block, used by every transform
   const LdsColumn<TK> map{col.c};
   const long ix = col.g / NH, iz = col.g % NH;
   auto at = [&](int m) { return ix * (long)N * NH + (long)(t + m * NT) * NH + iz; };
@@ -149,27 +151,27 @@ __global__ void __launch_bounds__((StridedShape<T, N, PL>::THREADS))
     load(j, v);
 #pragma unroll
     for (int m = 0; m < P; ++m) w[m] = mul_i(ky(m) * v[m]);
-    FFT::run(v, lds, map, t, tw);
+    FFT::run(v, lds, map, t, twr);
     store(j, v);
-    FFT::run(w, lds, map, t, tw);
+    FFT::run(w, lds, map, t, twr);
     store(6 + j, w);
   }
 #pragma unroll 1
   for (int j = 3; j < 6; ++j) {
     load(j, v);
-    FFT::run(v, lds, map, t, tw);
+    FFT::run(v, lds, map, t, twr);
     store(j, v);
   }
   // adjoint: g_x (A9) -> A9, and tau_bar_xy = A13 - 2i Ky g_x
   load(9, v);
 #pragma unroll
   for (int m = 0; m < P; ++m) w[m] = mul_mis(T(2) * ky(m), v[m]);
-  FFT::run(v, lds, map, t, tw);
+  FFT::run(v, lds, map, t, twr);
   store(9, v);
   load(13, v);
 #pragma unroll
   for (int m = 0; m < P; ++m) v[m] = v[m] + w[m];
-  FFT::run(v, lds, map, t, tw);
+  FFT::run(v, lds, map, t, twr);
   store(13, v);
   // g_y (A10) -> A10, tau_bar_yy = -2i Ky g_y -> A15 ; g_z (A11) -> A11, -2i Ky g_z -> A16
 #pragma unroll 1
@@ -177,16 +179,16 @@ __global__ void __launch_bounds__((StridedShape<T, N, PL>::THREADS))
     load(10 + j, v);
 #pragma unroll
     for (int m = 0; m < P; ++m) w[m] = mul_mis(T(2) * ky(m), v[m]);
-    FFT::run(v, lds, map, t, tw);
+    FFT::run(v, lds, map, t, twr);
     store(10 + j, v);
-    FFT::run(w, lds, map, t, tw);
+    FFT::run(w, lds, map, t, twr);
     store(15 + j, w);
   }
   // tau_bar_xx (A12), tau_bar_xz Kx part (A14) in place
 #pragma unroll 1
   for (int f = 12; f <= 14; f += 2) {
     load(f, v);
-    FFT::run(v, lds, map, t, tw);
+    FFT::run(v, lds, map, t, twr);
     store(f, v);
   }
 }
@@ -240,14 +242,16 @@ __global__ void __launch_bounds__(ZLines<PL>::THREADS)
     k_adj_z(cpx<T>* A, long C, T scale, T coeff, T dcoeff, double* cs_part, const cpx<T>* __restrict__ tw) {
   constexpr int P = PL::P, NT = PL::NT, NH = N / 2 + 1, KH = P / 2 + 1;
   constexpr int LPB = ZLines<PL>::LPB;
-  using IFFT = LineFFTP<T, PL, +1>;
-  using FFT = LineFFTP<T, PL, -1>;
+  constexpr bool WV = WaveLocal<PL>::value;  // each line inside one wavefront: no block barriers
+  using IFFT = LineFFTP<T, PL, +1, WV>;
+  using FFT = LineFFTP<T, PL, -1, WV>;
   __shared__ cpx<T> lds[LPB * LdsLine<T>::stride(N)];
   const int line = threadIdx.x / NT, t = threadIdx.x % NT;
   const long row = (long)blockIdx.x * LPB + line;
   const bool valid = row < (long)N * N;
   const long base = (valid ? row : 0) * NH;
   const LdsLine<T> map{line * LdsLine<T>::stride(N)};
+  const TwRegs<T, PL> twr(tw, t);  // read once, used by all 17 transforms
   cpx<T>* w[17];
 #pragma unroll
   for (int i = 0; i < 17; ++i) w[i] = A + i * C + base;
@@ -256,7 +260,7 @@ __global__ void __launch_bounds__(ZLines<PL>::THREADS)
   cpx<T> v[P];
 #define SLES_INV(PA, DA, PB, DB, RA, RB)                           \
   load_pair<T, N, P, NT>(v, PA, DA, PB, DB, t, valid);             \
-  IFFT::run(v, lds, map, t, tw);                                   \
+  IFFT::run(v, lds, map, t, twr);                                   \
   _Pragma("unroll") for (int m = 0; m < P; ++m) {                  \
     RA[m] = scale * v[m].x;                                        \
     RB[m] = scale * v[m].y;                                        \
@@ -272,7 +276,7 @@ __global__ void __launch_bounds__(ZLines<PL>::THREADS)
   T nb0[P], nb1[P], nb2[P], t11[P], t12[P], t22[P], t13[P], t23[P], t33[P];
 #define SLES_AINV(A1, A2, B1, B2, RA, RB)                          \
   load_adj_pair<T, N, P, NT>(v, A1, A2, B1, B2, t);                \
-  IFFT::run(v, lds, map, t, tw);                                   \
+  IFFT::run(v, lds, map, t, twr);                                   \
   _Pragma("unroll") for (int m = 0; m < P; ++m) {                  \
     RA[m] = v[m].x;                                                \
     RB[m] = v[m].y;                                                \
@@ -392,8 +396,8 @@ __global__ void __launch_bounds__(ZLines<PL>::THREADS)
   auto kok = [&](int m) { return t + m * NT <= N / 2; };
 #define SLES_FWD(QA, QB)                                               \
   _Pragma("unroll") for (int m = 0; m < P; ++m) v[m] = {QA[m], QB[m]}; \
-  FFT::run(v, lds, map, t, tw);                                        \
-  split_pair<T, N, P, NT>(v, lds, map, t, Ah, Bh);
+  FFT::run(v, lds, map, t, twr);                                        \
+  split_pair<T, N, P, NT, WV>(v, lds, map, t, Ah, Bh);
   SLES_FWD(a[2][0], a[2][1])
 #pragma unroll
   for (int m = 0; m < KH; ++m) {
@@ -449,6 +453,7 @@ __global__ void __launch_bounds__((StridedShape<T, N, PL>::THREADS))
   __shared__ cpx<T> lds[N * TK];
   const Cols<N, TK> col(xcd);
   const int t = threadIdx.x / TK;
+  const TwRegs<T, PL> twr(tw, t);  // read once per block, used by every transform
   const LdsColumn<TK> map{col.c};
   const long ix = col.g / NH, iz = col.g % NH;
   auto at = [&](int m) { return ix * (long)N * NH + (long)(t + m * NT) * NH + iz; };
@@ -457,12 +462,12 @@ __global__ void __launch_bounds__((StridedShape<T, N, PL>::THREADS))
   for (int j = 0; j < 3; ++j) {
 #pragma unroll
     for (int m = 0; m < P; ++m) v[m] = A[(6 + j) * C + at(m)];
-    FFT::run(v, lds, map, t, tw);
+    FFT::run(v, lds, map, t, twr);
 #pragma unroll
     for (int m = 0; m < P; ++m) acc[m] = mul_mis((T)kfreq(t + m * NT, N), v[m]);
 #pragma unroll
     for (int m = 0; m < P; ++m) v[m] = A[j * C + at(m)];
-    FFT::run(v, lds, map, t, tw);
+    FFT::run(v, lds, map, t, twr);
     if (col.valid)
 #pragma unroll
       for (int m = 0; m < P; ++m) A[j * C + at(m)] = v[m] + acc[m];
@@ -471,7 +476,7 @@ __global__ void __launch_bounds__((StridedShape<T, N, PL>::THREADS))
   for (int j = 3; j < 6; ++j) {
 #pragma unroll
     for (int m = 0; m < P; ++m) v[m] = A[j * C + at(m)];
-    FFT::run(v, lds, map, t, tw);
+    FFT::run(v, lds, map, t, twr);
     if (col.valid)
 #pragma unroll
       for (int m = 0; m < P; ++m) A[j * C + at(m)] = v[m];
@@ -497,6 +502,7 @@ __global__ void __launch_bounds__((StridedShape<T, N, PL>::THREADS))
   __shared__ cpx<T> lds[N * TK];
   const Cols<N, TK> col(xcd);
   const int t = threadIdx.x / TK;
+  const TwRegs<T, PL> twr(tw, t);  // read once per block, used by every transform
   const LdsColumn<TK> map{col.c};
   auto at = [&](int m) { return (long)(t + m * NT) * Cols<N, TK>::M + col.g; };
   const int ky = kfreq((int)(col.g / NH), N), kz = (int)(col.g % NH);
@@ -507,12 +513,12 @@ __global__ void __launch_bounds__((StridedShape<T, N, PL>::THREADS))
   for (int j = 0; j < 3; ++j) {
 #pragma unroll
     for (int m = 0; m < P; ++m) v[m] = A[(3 + j) * C + at(m)];
-    FFT::run(v, lds, map, t, tw);
+    FFT::run(v, lds, map, t, twr);
 #pragma unroll
     for (int m = 0; m < P; ++m) r[m] = mul_mis((T)kfreq(t + m * NT, N), v[m]);
 #pragma unroll
     for (int m = 0; m < P; ++m) v[m] = A[j * C + at(m)];
-    FFT::run(v, lds, map, t, tw);
+    FFT::run(v, lds, map, t, twr);
     if (col.valid) {
 #pragma unroll
       for (int m = 0; m < P; ++m) {
@@ -575,10 +581,10 @@ int adjoint_t(const sles_plan_s* p, const sles_step_params* prm, const void* y, 
   using D = LinePlan<N>;
   if constexpr (HasP4<N>::value) {
     using Q = LinePlan4<N>;
-    // the adjoint z kernel holds 21 physical lines per thread: radix-4 in FP64
-    using ZP = typename std::conditional<sizeof(T) == 8, Q, D>::type;
-    if (p->tune_strided) return adjoint_impl<T, N, Q, ZP>(p, prm, y, mu, out, accumulate, linear, dcoeff, cs_grad, work, s);
-    return adjoint_impl<T, N, D, ZP>(p, prm, y, mu, out, accumulate, linear, dcoeff, cs_grad, work, s);
+    // the adjoint z kernel holds 21 physical lines per thread: radix-4 plan
+    // (half the registers of radix-8) in both precisions
+    if (p->tune_strided) return adjoint_impl<T, N, Q, Q>(p, prm, y, mu, out, accumulate, linear, dcoeff, cs_grad, work, s);
+    return adjoint_impl<T, N, D, Q>(p, prm, y, mu, out, accumulate, linear, dcoeff, cs_grad, work, s);
   }
   return adjoint_impl<T, N, D, D>(p, prm, y, mu, out, accumulate, linear, dcoeff, cs_grad, work, s);
 }

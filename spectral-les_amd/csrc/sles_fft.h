@@ -32,6 +32,26 @@ __device__ __forceinline__ void block_sync() {
   asm volatile("" ::: "memory");
 }
 
+// Synchronisation of one wavefront's LDS traffic, for exchanges whose lines
+// all live inside one wavefront.  LDS operations of a wavefront execute in
+// program order, so only the compiler has to be kept from moving loads and
+// stores across this point (wavefront-scope fences plus "memory" clobbers);
+// no s_barrier is issued and the other waves of the block run on.
+__device__ __forceinline__ void wave_sync() {
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  asm volatile("" ::: "memory");
+}
+template <bool WAVE>
+__device__ __forceinline__ void line_sync() {
+  if constexpr (WAVE)
+    wave_sync();
+  else
+    block_sync();
+}
+
 template <typename T>
 struct alignas(2 * sizeof(T)) cpx {
   T x, y;
@@ -311,7 +331,7 @@ __device__ __forceinline__ void line_stage(cpx<T> (&v)[P], int t, const TW& tw) 
   }
 }
 
-template <typename T, int N, int P, int R, int Ns, class Map>
+template <typename T, int N, int P, int R, int Ns, bool WAVE, class Map>
 __device__ __forceinline__ void line_exchange(cpx<T> (&v)[P], cpx<T>* lds, const Map& map, int t) {
   constexpr int NT = N / P;
   constexpr int Q = P / R;
@@ -323,45 +343,50 @@ __device__ __forceinline__ void line_exchange(cpx<T> (&v)[P], cpx<T>* lds, const
 #pragma unroll
     for (int r = 0; r < R; ++r) lds[map(base + r * Ns)] = v[q + r * Q];
   }
-  block_sync();
+  line_sync<WAVE>();
 #pragma unroll
   for (int m = 0; m < P; ++m) v[m] = lds[map(t + m * NT)];
-  block_sync();
+  line_sync<WAVE>();
 }
 
-template <typename T, int N, int P, int DIR, int Ns, int OFF, int... Rs>
+template <typename T, int N, int P, int DIR, bool WAVE, int Ns, int OFF, int... Rs>
 struct LineStages;
 
-template <typename T, int N, int P, int DIR, int Ns, int OFF, int R>
-struct LineStages<T, N, P, DIR, Ns, OFF, R> {
+template <typename T, int N, int P, int DIR, bool WAVE, int Ns, int OFF, int R>
+struct LineStages<T, N, P, DIR, WAVE, Ns, OFF, R> {
   template <class Map, class TW>
   __device__ __forceinline__ static void run(cpx<T> (&v)[P], cpx<T>* lds, const Map& map, int t, const TW& tw) {
     line_stage<T, N, P, DIR, R, Ns, OFF>(v, t, tw);
   }
 };
 
-template <typename T, int N, int P, int DIR, int Ns, int OFF, int R, int R2, int... Rest>
-struct LineStages<T, N, P, DIR, Ns, OFF, R, R2, Rest...> {
+template <typename T, int N, int P, int DIR, bool WAVE, int Ns, int OFF, int R, int R2, int... Rest>
+struct LineStages<T, N, P, DIR, WAVE, Ns, OFF, R, R2, Rest...> {
   static constexpr int NEXT = OFF + (Ns > 1 ? (P / R) * (R - 1) : 0);
   template <class Map, class TW>
   __device__ __forceinline__ static void run(cpx<T> (&v)[P], cpx<T>* lds, const Map& map, int t, const TW& tw) {
     line_stage<T, N, P, DIR, R, Ns, OFF>(v, t, tw);
-    line_exchange<T, N, P, R, Ns>(v, lds, map, t);
-    LineStages<T, N, P, DIR, Ns * R, NEXT, R2, Rest...>::run(v, lds, map, t, tw);
+    line_exchange<T, N, P, R, Ns, WAVE>(v, lds, map, t);
+    LineStages<T, N, P, DIR, WAVE, Ns * R, NEXT, R2, Rest...>::run(v, lds, map, t, tw);
   }
 };
 
-template <typename T, class PL, int DIR>
+// WAVE: every line of the block lies inside one wavefront (NT <= 64 and
+// 64 % NT == 0, lines packed by thread index), so the exchanges need only
+// wavefront synchronisation.
+template <typename T, class PL, int DIR, bool WAVE = false>
 struct LineFFTP {
   static constexpr int N = PL::N;
   static constexpr int P = PL::P;
   static constexpr int NT = PL::NT;
+  static_assert(!WAVE || (NT <= 64 && 64 % NT == 0), "wavefront-local lines need NT | 64");
   template <int... Rs>
-  using stages = LineStages<T, N, P, DIR, 1, 0, Rs...>;
+  using stages = LineStages<T, N, P, DIR, WAVE, 1, 0, Rs...>;
   using S = typename PL::template apply<stages>;
   // Transform the line held by thread t (0 <= t < NT).  Every thread of the
-  // block must call this (it contains block barriers when the plan has more
-  // than one stage).  tw: the twiddle table, or a TwRegs<T, PL> of thread t.
+  // block (WAVE: of the wavefront) must call this: it contains barriers when
+  // the plan has more than one stage.  tw: the twiddle table, or a
+  // TwRegs<T, PL> of thread t.
   template <class Map>
   __device__ __forceinline__ static void run(cpx<T> (&v)[P], cpx<T>* lds, const Map& map, int t,
                                              const cpx<T>* __restrict__ tw) {
@@ -372,6 +397,12 @@ struct LineFFTP {
                                              const TwRegs<T, PL>& tw) {
     S::run(v, lds, map, t, tw);
   }
+};
+
+// Lines of this plan fit inside one wavefront.
+template <class PL>
+struct WaveLocal {
+  static constexpr bool value = PL::NT <= 64 && 64 % PL::NT == 0;
 };
 
 template <typename T, int N, int DIR>

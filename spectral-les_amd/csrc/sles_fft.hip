@@ -46,6 +46,7 @@ __global__ void __launch_bounds__(ZShape<N>::THREADS)
   __shared__ cpx<T> lds[S::LPB * LdsLine<T>::stride(N)];
   const LdsLine<T> map{(int)(threadIdx.x / S::NT) * LdsLine<T>::stride(N)};
   const int line = threadIdx.x / NT, t = threadIdx.x % NT;
+  const TwRegs<T, LinePlan<N>> twr(tw, t);  // issued together with the data loads
   const long ra = 2 * ((long)blockIdx.x * S::LPB + line), rb = ra + 1;
   const bool va = ra < nrows, vb = rb < nrows;
   const long rla = va ? ra : 0, rlb = vb ? rb : 0;  // clamped rows (in bounds)
@@ -57,11 +58,11 @@ __global__ void __launch_bounds__(ZShape<N>::THREADS)
     v[m].x = va ? xa : T(0);
     v[m].y = vb ? xb : T(0);
   }
-  LineFFT<T, N, -1>::run(v, lds, map, t, tw);
+  LineFFTP<T, LinePlan<N>, -1, WaveLocal<LinePlan<N>>::value>::run(v, lds, map, t, twr);
   // Split Z = FFT(a + i b):  A_k = (Z_k + conj Z_{N-k})/2,  B_k = (Z_k - conj Z_{N-k})/(2i)
 #pragma unroll
   for (int m = 0; m < P; ++m) lds[map(t + m * NT)] = v[m];
-  block_sync();
+  line_sync<WaveLocal<LinePlan<N>>::value>();
 #pragma unroll
   for (int m = 0; m <= P / 2; ++m) {
     const int k = t + m * NT;
@@ -86,6 +87,7 @@ __global__ void __launch_bounds__(ZShape<N>::THREADS)
   __shared__ cpx<T> lds[S::LPB * LdsLine<T>::stride(N)];
   const LdsLine<T> map{(int)(threadIdx.x / S::NT) * LdsLine<T>::stride(N)};
   const int line = threadIdx.x / NT, t = threadIdx.x % NT;
+  const TwRegs<T, LinePlan<N>> twr(tw, t);  // issued together with the data loads
   const long ra = 2 * ((long)blockIdx.x * S::LPB + line), rb = ra + 1;
   const bool va = ra < nrows, vb = rb < nrows;
   const long rla = va ? ra : 0, rlb = vb ? rb : 0;  // clamped rows (in bounds)
@@ -106,7 +108,7 @@ __global__ void __launch_bounds__(ZShape<N>::THREADS)
     }
     v[m] = {a.x - b.y, a.y + b.x};  // a + i b
   }
-  LineFFT<T, N, +1>::run(v, lds, map, t, tw);
+  LineFFTP<T, LinePlan<N>, +1, WaveLocal<LinePlan<N>>::value>::run(v, lds, map, t, twr);
 #pragma unroll
   for (int m = 0; m < P; ++m) {
     const int z = t + m * NT;
@@ -127,6 +129,7 @@ __global__ void __launch_bounds__((StridedShape<T, N>::THREADS))
   constexpr int P = S::P, NT = S::NT, TK = S::TK;
   __shared__ cpx<T> lds[N * TK];
   const int c = threadIdx.x % TK, t = threadIdx.x / TK;
+  const TwRegs<T, LinePlan<N>> twr(tw, t);  // issued together with the data loads
   const long g = (long)blockIdx.x * TK + c;
   const bool valid = g < outer * M;
   const long o = valid ? g / M : 0, col = valid ? g % M : 0;
@@ -134,7 +137,7 @@ __global__ void __launch_bounds__((StridedShape<T, N>::THREADS))
   cpx<T> v[P];
 #pragma unroll
   for (int m = 0; m < P; ++m) v[m] = in[base + (long)(t + m * NT) * M];  // in bounds (clamped); unused if !valid
-  LineFFT<T, N, DIR>::run(v, lds, LdsColumn<TK>{c}, t, tw);
+  LineFFT<T, N, DIR>::run(v, lds, LdsColumn<TK>{c}, t, twr);
   if (valid) {
 #pragma unroll
     for (int m = 0; m < P; ++m) out[base + (long)(t + m * NT) * M] = scale * v[m];

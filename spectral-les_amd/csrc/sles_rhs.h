@@ -261,12 +261,12 @@ __device__ __forceinline__ void load_pair32(cpx<T> (&v)[P], const cpx<T>* A, boo
 // After a forward FFT of a + i b held in v (natural layout), split into the
 // half-spectra A_k, B_k of a and b for this thread's bins k = t + m*NT <= N/2,
 // m = 0..P/2.  Uses the line's LDS row; ends with a barrier.
-template <typename T, int N, int P, int NT, class Map>
+template <typename T, int N, int P, int NT, bool WAVE = false, class Map>
 __device__ __forceinline__ void split_pair(const cpx<T> (&v)[P], cpx<T>* lds, const Map& map, int t,
                                            cpx<T> (&A)[P / 2 + 1], cpx<T> (&B)[P / 2 + 1]) {
 #pragma unroll
   for (int m = 0; m < P; ++m) lds[map(t + m * NT)] = v[m];
-  block_sync();
+  line_sync<WAVE>();
 #pragma unroll
   for (int m = 0; m <= P / 2; ++m) {
     const int k = t + m * NT;
@@ -276,7 +276,7 @@ __device__ __forceinline__ void split_pair(const cpx<T> (&v)[P], cpx<T>* lds, co
     A[m] = T(0.5) * (zk + zn);
     B[m] = {T(0.5) * d.y, T(-0.5) * d.x};
   }
-  block_sync();
+  line_sync<WAVE>();
 }
 
 template <typename T>
@@ -306,8 +306,9 @@ __global__ void __launch_bounds__(ZLines<PL>::THREADS)
     k_zles(cpx<T>* W, long C, T scale, T coeff, double* umax_part, const cpx<T>* __restrict__ tw) {
   constexpr int P = PL::P, NT = PL::NT, NH = N / 2 + 1, KH = P / 2 + 1;
   constexpr int LPB = ZLines<PL>::LPB;
-  using IFFT = LineFFTP<T, PL, +1>;
-  using FFT = LineFFTP<T, PL, -1>;
+  constexpr bool WV = WaveLocal<PL>::value;  // each line inside one wavefront: no block barriers
+  using IFFT = LineFFTP<T, PL, +1, WV>;
+  using FFT = LineFFTP<T, PL, -1, WV>;
   __shared__ cpx<T> lds[LPB * LdsLine<T>::stride(N)];
   const int line = threadIdx.x / NT, t = threadIdx.x % NT;
   const long row = (long)blockIdx.x * LPB + line;
@@ -380,7 +381,7 @@ __global__ void __launch_bounds__(ZLines<PL>::THREADS)
 #define SLES_FWD(QA, QB)                                     \
   _Pragma("unroll") for (int m = 0; m < P; ++m) v[m] = {QA[m], QB[m]}; \
   FFT::run(v, lds, map, t, twr);                             \
-  split_pair<T, N, P, NT>(v, lds, map, t, A, B);
+  split_pair<T, N, P, NT, WV>(v, lds, map, t, A, B);
   SLES_FWD(q4, q5)  // tau_xy, tau_xz
 #pragma unroll
   for (int m = 0; m < KH; ++m) {
@@ -429,8 +430,9 @@ __global__ void __launch_bounds__(ZLines<PL>::THREADS)
     k_zns(cpx<T>* W, long C, T scale, double* umax_part, const cpx<T>* __restrict__ tw) {
   constexpr int P = PL::P, NT = PL::NT, NH = N / 2 + 1, KH = P / 2 + 1;
   constexpr int LPB = ZLines<PL>::LPB;
-  using IFFT = LineFFTP<T, PL, +1>;
-  using FFT = LineFFTP<T, PL, -1>;
+  constexpr bool WV = WaveLocal<PL>::value;  // each line inside one wavefront: no block barriers
+  using IFFT = LineFFTP<T, PL, +1, WV>;
+  using FFT = LineFFTP<T, PL, -1, WV>;
   __shared__ cpx<T> lds[LPB * LdsLine<T>::stride(N)];
   const int line = threadIdx.x / NT, t = threadIdx.x % NT;
   const long row = (long)blockIdx.x * LPB + line;
@@ -468,7 +470,7 @@ __global__ void __launch_bounds__(ZLines<PL>::THREADS)
 #pragma unroll
   for (int m = 0; m < P; ++m) v[m] = {n0[m], n1[m]};
   FFT::run(v, lds, map, t, twr);
-  split_pair<T, N, P, NT>(v, lds, map, t, A, B);
+  split_pair<T, N, P, NT, WV>(v, lds, map, t, A, B);
 #pragma unroll
   for (int m = 0; m < KH; ++m)
     if (valid && t + m * NT <= N / 2) {
@@ -478,7 +480,7 @@ __global__ void __launch_bounds__(ZLines<PL>::THREADS)
 #pragma unroll
   for (int m = 0; m < P; ++m) v[m] = {n2[m], T(0)};
   FFT::run(v, lds, map, t, twr);
-  split_pair<T, N, P, NT>(v, lds, map, t, A, B);
+  split_pair<T, N, P, NT, WV>(v, lds, map, t, A, B);
 #pragma unroll
   for (int m = 0; m < KH; ++m)
     if (valid && t + m * NT <= N / 2) st32(w[2], row0 + t + m * NT, A[m]);

@@ -226,20 +226,40 @@ class _Advance(torch.autograd.Function):
         return lam.to(ctx.in_dtype), gCs, None
 
 
+_SHELLS = {}
+
+
+def _shell_layout(N, kmax, device):
+    """Modes of the (N, N, N/2+1) half spectrum with unit-shell index
+    floor(|k| + 1/2) in 1..kmax, sorted by shell: (flat indices, rfft weights
+    w = 2 except on the kz = 0, N/2 planes, modes per shell).  Cached per
+    (N, kmax, device)."""
+    key = (N, kmax, str(device))
+    if key not in _SHELLS:
+        k = torch.fft.fftfreq(N, 1.0 / N, dtype=torch.float64, device=device)
+        kz = torch.arange(N // 2 + 1, dtype=torch.float64, device=device)
+        kmag = torch.sqrt(k[:, None, None] ** 2 + k[None, :, None] ** 2 + kz[None, None, :] ** 2)
+        shell = torch.floor(kmag + 0.5).to(torch.int64).flatten()
+        sel = ((shell >= 1) & (shell <= kmax)).nonzero().flatten()
+        order = torch.argsort(shell[sel], stable=True)
+        idx = sel[order]
+        w = torch.full((N, N, N // 2 + 1), 2.0, dtype=torch.float64, device=device)
+        w[..., 0] = 1.0
+        w[..., -1] = 1.0
+        counts = torch.bincount(shell[idx] - 1, minlength=kmax)
+        _SHELLS[key] = (idx, w.flatten()[idx].contiguous(), counts)
+    return _SHELLS[key]
+
+
 def shell_spectrum(U_hat, N, kmax=40):
     """E(k) over unit shells |k| in [k - 1/2, k + 1/2), k = 1..kmax:
-    sum (1/2) w |U_hat|^2 / N^6 with the rfft doubling w (torch ops, differentiable)."""
-    dev = U_hat.device
-    k = torch.fft.fftfreq(N, 1.0 / N, dtype=torch.float64, device=dev)
-    kz = torch.arange(N // 2 + 1, dtype=torch.float64, device=dev)
-    kmag = torch.sqrt(k[:, None, None] ** 2 + k[None, :, None] ** 2 + kz[None, None, :] ** 2)
-    w = torch.full_like(kmag, 2.0)
-    w[..., 0] = 1.0
-    w[..., -1] = 1.0
-    e = 0.5 * w * (U_hat.real.double() ** 2 + U_hat.imag.double() ** 2).sum(0) / float(N) ** 6
-    idx = torch.floor(kmag + 0.5).to(torch.int64).flatten()
-    E = torch.zeros(int(idx.max()) + 1, dtype=torch.float64, device=dev).index_add(0, idx, e.flatten())
-    return E[1 : kmax + 1]
+    sum (1/2) w |U_hat|^2 / N^6 with the rfft doubling w.  Differentiable torch
+    ops: a gather of the modes sorted by shell and a segmented sum per shell
+    (no atomics), FP64."""
+    idx, w, counts = _shell_layout(N, kmax, U_hat.device)
+    sq = (U_hat.real.double() ** 2 + U_hat.imag.double() ** 2).sum(0).flatten()
+    e = (0.5 / float(N) ** 6) * w * sq[idx]
+    return torch.segment_reduce(e, "sum", lengths=counts)
 
 
 def spectrum_loss(U_hat, target_E, N, kmax=40):
