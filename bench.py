@@ -162,6 +162,53 @@ def measured_traffic(kernel, N, precision):
     return None
 
 
+def tgv_gpu(N=64, steps=200, warmup=5):
+    """The CPU-runnable Taylor-Green case: 64^3, nu = 1/1600, dt = 0.01, RK4,
+    no LES, FP64; fused steps timed with events (U and curl written each step)."""
+    import numpy as np
+    import torch
+
+    from sles import fused
+    from sles.initial import initial_velocity
+    from sles.initialize_domain import InitializeDomain
+
+    d = InitializeDomain(N=N, nu=1 / 1600, time_stepSize=0.01, method="RK4")
+    x = np.arange(N) * 2 * np.pi / N
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    initial_velocity(d, np.sin(X) * np.cos(Y) * np.cos(Z), -np.cos(X) * np.sin(Y) * np.cos(Z), np.zeros_like(X))
+    d.U_hat1.copy_(d.U_hat)
+    for _ in range(warmup):
+        fused.rk4_step(d)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        fused.rk4_step(d)
+    e1.record()
+    e1.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    return {"steps_per_s": round(1e3 / ms, 1), "ms_per_step": round(ms, 4), "steps": steps}
+
+
+def tgv_cpu(N=64, steps=20):
+    """The same Taylor-Green case on the oracle (numpy FP64, one process)."""
+    import numpy as np
+
+    from oracle import sles_oracle as O
+
+    o = O.Domain(N=N, nu=1 / 1600, time_stepSize=0.01, method="RK4")
+    x, y, z = O.grid(N)
+    O.initial_velocity(o, np.sin(x) * np.cos(y) * np.cos(z), -np.cos(x) * np.sin(y) * np.cos(z), np.zeros_like(x))
+    o.U_hat1[:] = o.U_hat
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        O.rk4_step(o)
+    t = time.perf_counter() - t0
+    return {"steps_per_s": round(steps / t, 3), "steps": steps, "cores": 1,
+            "sample": f"{steps} RK4 steps of the 64^3 Taylor-Green case, numpy pocketfft, {t:.2f} s"}
+
+
 def roofline_of(res, N, precision, les=True):
     R, C = field_bytes(N, precision)
     kern = res["kernels"]
@@ -283,8 +330,12 @@ def main(argv=None):
         extra["fp32_speedup_vs_fp64"] = round(v32 / value, 3)
         extra["kernels_fp32"] = roofline_of(r32, N, "fp32")[1]
         log(f"[bench] fp32 {v32:.2f} steps/s")
+    log("[bench] 64^3 Taylor-Green (no LES) FP64")
+    extra["taylor_green_64_fp64"] = tgv_gpu()
+    log(f"[bench] tgv64 {extra['taylor_green_64_fp64']['steps_per_s']} steps/s")
     cpu = None
     if not a.no_cpu:
+        extra["taylor_green_64_cpu_oracle"] = tgv_cpu()
         log("[bench] CPU baseline (oracle, one RK4 step)")
         cpu = cpu_baseline(N)
         log(f"[bench] cpu {cpu['value']} steps/s on {cpu['cores']} cores")

@@ -38,6 +38,10 @@ namespace sles {
 
 enum { EPI_RK4 = 0, EPI_RK3 = 1, EPI_ETD2 = 2, EPI_RHS = 3, EPI_NONLIN = 4 };
 
+#ifndef SLES_PF
+#define SLES_PF 1
+#endif
+
 // Element index inside one spectral field and the accessors of the strided
 // passes.
 using eidx = long;
@@ -236,6 +240,42 @@ __device__ __forceinline__ void load_pair(cpx<T> (&v)[P], const cpx<T>* A, bool 
   }
 }
 
+// Raw loads of a pair (bins kk = min(k, N-k)) and the completion step, so
+// that the next pair's loads can be issued before the current transform.
+template <typename T, int P>
+struct PairRaw {
+  cpx<T> a[P], b[P];
+};
+template <typename T, int N, int P, int NT>
+__device__ __forceinline__ void pair_issue(PairRaw<T, P>& r, const cpx<T>* A, const cpx<T>* B, unsigned row0, int t) {
+#pragma unroll
+  for (int m = 0; m < P; ++m) {
+    const int k = t + m * NT;
+    const int kk = k <= N / 2 ? k : N - k;
+    r.a[m] = ld32(A, row0 + kk);
+    r.b[m] = ld32(B, row0 + kk);
+  }
+}
+template <typename T, int N, int P, int NT>
+__device__ __forceinline__ void pair_finish(cpx<T> (&v)[P], const PairRaw<T, P>& r, bool dzA, bool dzB, int t) {
+#pragma unroll
+  for (int m = 0; m < P; ++m) {
+    const int k = t + m * NT;
+    const int kk = k <= N / 2 ? k : N - k;
+    cpx<T> a = r.a[m], b = r.b[m];
+    if (dzA) a = mul_i((T)kk * a);
+    if (dzB) b = mul_i((T)kk * b);
+    if (k > N / 2) {
+      a = conj(a);
+      b = conj(b);
+    } else if (k == 0 || k == N / 2) {
+      a.y = T(0);
+      b.y = T(0);
+    }
+    v[m] = {a.x - b.y, a.y + b.x};
+  }
+}
+
 // Same with uniform field bases A, B and the row's first element `row0`.
 template <typename T, int N, int P, int NT>
 __device__ __forceinline__ void load_pair32(cpx<T> (&v)[P], const cpx<T>* A, bool dzA, const cpx<T>* B, bool dzB,
@@ -323,6 +363,26 @@ __global__ void __launch_bounds__(ZLines<PL>::THREADS)
   // ---- inverse: 6 complex transforms -> 12 physical fields (scaled 1/N^3)
   T u0[P], u1[P], u2[P], a11[P], a12[P], a13[P], a21[P], a22[P], a23[P], a31[P], a32[P], a33[P];
   cpx<T> v[P];
+#if SLES_PF
+  // pair i+1 is loaded while pair i is transformed
+  PairRaw<T, P> pr;
+  pair_issue<T, N, P, NT>(pr, w[0], w[1], row0, t);
+#define SLES_INV(DA, DB, NA, NB, RA, RB)                           \
+  pair_finish<T, N, P, NT>(v, pr, DA, DB, t);                      \
+  if (NA) pair_issue<T, N, P, NT>(pr, NA, NB, row0, t);            \
+  IFFT::run(v, lds, map, t, twr);                                  \
+  _Pragma("unroll") for (int m = 0; m < P; ++m) {                  \
+    RA[m] = scale * v[m].x;                                        \
+    RB[m] = scale * v[m].y;                                        \
+  }
+  SLES_INV(false, false, w[2], w[0], u0, u1)
+  SLES_INV(false, true, w[1], w[2], u2, a31)
+  SLES_INV(true, true, w[3], w[4], a32, a33)
+  SLES_INV(false, false, w[5], w[6], a11, a12)
+  SLES_INV(false, false, w[7], w[8], a13, a21)
+  SLES_INV(false, false, (cpx<T>*)nullptr, (cpx<T>*)nullptr, a22, a23)
+#undef SLES_INV
+#else
 #define SLES_INV(A, DA, B, DB, RA, RB)                             \
   load_pair32<T, N, P, NT>(v, A, DA, B, DB, row0, t);              \
   IFFT::run(v, lds, map, t, twr);                                  \
@@ -337,6 +397,7 @@ __global__ void __launch_bounds__(ZLines<PL>::THREADS)
   SLES_INV(w[5], false, w[6], false, a13, a21)
   SLES_INV(w[7], false, w[8], false, a22, a23)
 #undef SLES_INV
+#endif
 
   // ---- pointwise: omega, u x omega, nu_R, tau = nu_R S_ij, max(u)
   T q0[P], q1[P], q2[P], q3[P], q4[P], q5[P], q6[P], q7[P], q8[P];
@@ -673,10 +734,26 @@ __global__ void __launch_bounds__((StridedShape<T, N, PL, 512>::THREADS))
   // a0..a2: S_i = F_x(W_i) (+ 2i Kx F_x(tau_xi) with LES).  Named arrays and
   // a rolled field loop with uniform branches keep them in registers.
   cpx<T> a0[P], a1[P], a2[P], v[P];
+#if SLES_PF
+  // next field's loads are issued before the current field's transform
+  constexpr int NF = LES ? 6 : 3;
+  cpx<T> nx[P];
+#pragma unroll
+  for (int m = 0; m < P; ++m) nx[m] = ldx(W, at(m));
+#endif
 #pragma unroll 1
   for (int f = 0; f < (LES ? 6 : 3); ++f) {
+#if SLES_PF
+#pragma unroll
+    for (int m = 0; m < P; ++m) v[m] = nx[m];
+    if (f + 1 < NF) {
+#pragma unroll
+      for (int m = 0; m < P; ++m) nx[m] = ldx(W + (f + 1) * C, at(m));
+    }
+#else
 #pragma unroll
     for (int m = 0; m < P; ++m) v[m] = ldx(W + f * C, at(m));
+#endif
     FFT::run(v, lds, map, t, twr);
     if (f < 3) {
       if (f == 0) {
