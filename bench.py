@@ -191,6 +191,46 @@ def tgv_gpu(N=64, steps=200, warmup=5):
     return {"steps_per_s": round(1e3 / ms, 1), "ms_per_step": round(ms, 4), "steps": steps}
 
 
+def fft_rates(N, precision, reps=10):
+    """Generic 3D transforms (sles_fft_r2c / sles_fft_c2r) of a batch of 3
+    fields: ms per call and algorithmic GB/s, 3 (R + 5 C) bytes per call (z
+    pass R in / C out, y and x passes C in / C out, each field once)."""
+    import torch
+
+    from sles import _lib
+
+    prec = _lib.SLES_FP64 if precision == "fp64" else _lib.SLES_FP32
+    rdt = torch.float64 if precision == "fp64" else torch.float32
+    cdt = torch.complex128 if precision == "fp64" else torch.complex64
+    plan = _lib.Plan(N, prec, torch.device("cuda"))
+    u = torch.randn(3, N, N, N, dtype=rdt, device="cuda")
+    U = torch.empty(3, N, N, N // 2 + 1, dtype=cdt, device="cuda")
+    s = torch.cuda.current_stream().cuda_stream
+    R, C = field_bytes(N, precision)
+    out = {}
+    for name, xcd, fn in (
+        ("r2c", 0, lambda: _lib.call("sles_fft_r2c", plan.handle, u.data_ptr(), U.data_ptr(), 3, 1.0, s)),
+        ("c2r", 0, lambda: _lib.call("sles_fft_c2r", plan.handle, U.data_ptr(), u.data_ptr(), U.data_ptr(), 3,
+                                     1.0 / N ** 3, s)),
+        ("r2c_xcd", 1, lambda: _lib.call("sles_fft_r2c", plan.handle, u.data_ptr(), U.data_ptr(), 3, 1.0, s)),
+        ("c2r_xcd", 1, lambda: _lib.call("sles_fft_c2r", plan.handle, U.data_ptr(), u.data_ptr(), U.data_ptr(),
+                                         3, 1.0 / N ** 3, s)),
+    ):
+        plan.set_tuning(xcd=xcd)
+        fn()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        e1.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        out[name] = {"ms": round(ms, 4), "algorithmic_GBps": round(3 * (R + 5 * C) / (ms * 1e-3) / 1e9, 1)}
+    return out
+
+
 def tgv_cpu(N=64, steps=20):
     """The same Taylor-Green case on the oracle (numpy FP64, one process)."""
     import numpy as np
@@ -330,6 +370,8 @@ def main(argv=None):
         extra["fp32_speedup_vs_fp64"] = round(v32 / value, 3)
         extra["kernels_fp32"] = roofline_of(r32, N, "fp32")[1]
         log(f"[bench] fp32 {v32:.2f} steps/s")
+    extra["fft_3d_batch3"] = {f"{N}_{pr}": fft_rates(N, pr) for pr in ("fp64", "fp32")}
+    log(f"[bench] generic 3D FFT: {extra['fft_3d_batch3']}")
     log("[bench] 64^3 Taylor-Green (no LES) FP64")
     extra["taylor_green_64_fp64"] = tgv_gpu()
     log(f"[bench] tgv64 {extra['taylor_green_64_fp64']['steps_per_s']} steps/s")

@@ -120,27 +120,38 @@ __global__ void __launch_bounds__(ZShape<N>::THREADS)
 // ---------------------------------------------------------------------------
 // Strided complex pass: batch of fields, each viewed as (outer, N, M) with the
 // transform along the middle axis.  Column g in [0, outer*M) -> (g / M, g % M).
+// One block per tile of TK columns runs every field of the batch, the next
+// field's loads issued before the current field's transform.
 // ---------------------------------------------------------------------------
 template <typename T, int N, int DIR>
 __global__ void __launch_bounds__((StridedShape<T, N>::THREADS))
-    k_strided(const cpx<T>* in, cpx<T>* out, long outer, long M, long fstride, T scale,
-              const cpx<T>* __restrict__ tw) {
+    k_strided(const cpx<T>* in, cpx<T>* out, long outer, long M, long fstride, int nfields, T scale,
+              const cpx<T>* __restrict__ tw, int xcd) {
   using S = StridedShape<T, N>;
   constexpr int P = S::P, NT = S::NT, TK = S::TK;
   __shared__ cpx<T> lds[N * TK];
   const int c = threadIdx.x % TK, t = threadIdx.x / TK;
   const TwRegs<T, LinePlan<N>> twr(tw, t);  // issued together with the data loads
-  const long g = (long)blockIdx.x * TK + c;
+  const long g = tile_id(xcd) * TK + c;
   const bool valid = g < outer * M;
   const long o = valid ? g / M : 0, col = valid ? g % M : 0;
-  const long base = (long)blockIdx.y * fstride + o * (long)N * M + col;
-  cpx<T> v[P];
+  const long base = o * (long)N * M + col;
+  cpx<T> v[P], nx[P];
 #pragma unroll
-  for (int m = 0; m < P; ++m) v[m] = in[base + (long)(t + m * NT) * M];  // in bounds (clamped); unused if !valid
-  LineFFT<T, N, DIR>::run(v, lds, LdsColumn<TK>{c}, t, twr);
-  if (valid) {
+  for (int m = 0; m < P; ++m) nx[m] = in[base + (long)(t + m * NT) * M];  // in bounds (clamped); unused if !valid
+#pragma unroll 1
+  for (int f = 0; f < nfields; ++f) {
 #pragma unroll
-    for (int m = 0; m < P; ++m) out[base + (long)(t + m * NT) * M] = scale * v[m];
+    for (int m = 0; m < P; ++m) v[m] = nx[m];
+    if (f + 1 < nfields) {
+#pragma unroll
+      for (int m = 0; m < P; ++m) nx[m] = in[(f + 1) * fstride + base + (long)(t + m * NT) * M];
+    }
+    LineFFT<T, N, DIR>::run(v, lds, LdsColumn<TK>{c}, t, twr);
+    if (valid) {
+#pragma unroll
+      for (int m = 0; m < P; ++m) out[f * fstride + base + (long)(t + m * NT) * M] = scale * v[m];
+    }
   }
 }
 
@@ -175,13 +186,11 @@ static int launch_strided(const sles_plan_s* p, const void* in, void* out, long 
   using S = StridedShape<T, N>;
   const long blocks = (outer * M + S::TK - 1) / S::TK;
   if (dir < 0)
-    hipLaunchKernelGGL((k_strided<T, N, -1>), dim3((unsigned)blocks, nfields), dim3(S::THREADS), 0, s,
-                       (const cpx<T>*)in, (cpx<T>*)out, outer, M, fstride, (T)scale,
-                       (const cpx<T>*)p->tw);
+    hipLaunchKernelGGL((k_strided<T, N, -1>), dim3((unsigned)blocks), dim3(S::THREADS), 0, s, (const cpx<T>*)in,
+                       (cpx<T>*)out, outer, M, fstride, nfields, (T)scale, (const cpx<T>*)p->tw, p->tune_xcd);
   else
-    hipLaunchKernelGGL((k_strided<T, N, +1>), dim3((unsigned)blocks, nfields), dim3(S::THREADS), 0, s,
-                       (const cpx<T>*)in, (cpx<T>*)out, outer, M, fstride, (T)scale,
-                       (const cpx<T>*)p->tw);
+    hipLaunchKernelGGL((k_strided<T, N, +1>), dim3((unsigned)blocks), dim3(S::THREADS), 0, s, (const cpx<T>*)in,
+                       (cpx<T>*)out, outer, M, fstride, nfields, (T)scale, (const cpx<T>*)p->tw, p->tune_xcd);
   return check_launch("strided");
 }
 

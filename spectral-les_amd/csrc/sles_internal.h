@@ -18,7 +18,7 @@ struct sles_plan_s {
   int tune_strided = 0;  // x/y inverse and y forward passes
   int tune_xfwd = 0;     // x forward + stage update
   int tune_z = 0;        // fused z kernel
-  int tune_xcd = 0;      // XCD-aware tile order in the strided passes
+  int tune_xcd = 1;      // XCD-aware tile order in the strided passes (default on)
 };
 
 namespace sles {
@@ -51,6 +51,17 @@ __device__ __forceinline__ KIndex kindex(long e, int n, int nh) {
 __device__ __forceinline__ bool dealias_keep(KIndex k, int n) {
   const double kmax = 2.0 / 3.0 * (double)(n / 2 + 1);
   return (double)abs(k.kx) < kmax && (double)abs(k.ky) < kmax && (double)abs(k.kz) < kmax;
+}
+
+// Tile of this block in a strided pass.  With xcd != 0 (and a grid that is a
+// multiple of 8) consecutive tiles go to the same XCD: hardware dispatches
+// block b to XCD b mod 8, so tile (b mod 8) * (grid/8) + b/8 keeps
+// neighbouring columns -- which share cache lines on the y axis -- inside one
+// L2.
+__device__ __forceinline__ long tile_id(int xcd) {
+  const long b = blockIdx.x;
+  if (xcd && (gridDim.x & 7) == 0) return (b & 7) * (long)(gridDim.x >> 3) + (b >> 3);
+  return b;
 }
 
 // Shapes of the generic passes.

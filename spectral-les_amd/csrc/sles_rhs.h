@@ -61,15 +61,7 @@ struct StepArgs {
 // ---------------------------------------------------------------------------
 // Tile geometry of the strided passes (N * (N/2+1) columns, TK per block).
 // ---------------------------------------------------------------------------
-// With xcd != 0 (and a grid that is a multiple of 8) consecutive tiles go to
-// the same XCD: hardware dispatches block b to XCD b mod 8, so tile
-// (b mod 8) * (grid/8) + b/8 keeps neighbouring columns -- which share cache
-// lines on the y axis -- inside one L2.
-__device__ __forceinline__ long tile_id(int xcd) {
-  const long b = blockIdx.x;
-  if (xcd && (gridDim.x & 7) == 0) return (b & 7) * (long)(gridDim.x >> 3) + (b >> 3);
-  return b;
-}
+// Tiles are numbered by tile_id (sles_internal.h), XCD-aware when xcd != 0.
 
 template <int N, int TK>
 struct Cols {
@@ -123,38 +115,36 @@ __global__ void __launch_bounds__((StridedShape<T, N, PL>::THREADS))
         for (int m = 0; m < P; ++m) stx(W + (3 + j) * C, at(m), w[m]);
     }
   } else {
-#pragma unroll 1
-    for (int j = 0; j < 3; ++j) {
+    // U_hat and omega_hat = 1j * (K x U_hat) (math_formula.py:22-24) from one
+    // read of the three components: the curl components first (they need all
+    // three), then each component transformed in place, so the live set stays
+    // at four P-point arrays
+    cpx<T> u0[P], u1[P], u2[P];
 #pragma unroll
-      for (int m = 0; m < P; ++m) v[m] = ldx(uh + j * C, at(m));
-      FFT::run(v, lds, map, t, twr);
+    for (int m = 0; m < P; ++m) {
+      u0[m] = ldx(uh, at(m));
+      u1[m] = ldx(uh + C, at(m));
+      u2[m] = ldx(uh + 2 * C, at(m));
+    }
+    auto kx = [&](int m) { return (T)kfreq(t + m * NT, N); };
+    auto fft_store = [&](cpx<T>(&x)[P], int j) {
+      FFT::run(x, lds, map, t, twr);
       if (col.valid)
 #pragma unroll
-        for (int m = 0; m < P; ++m) stx(W + j * C, at(m), v[m]);
-    }
-    // omega_hat = 1j * (K x U_hat)  (math_formula.py:22-24)
-#pragma unroll 1
-    for (int j = 0; j < 3; ++j) {
+        for (int m = 0; m < P; ++m) stx(W + j * C, at(m), x[m]);
+    };
 #pragma unroll
-      for (int m = 0; m < P; ++m) {
-        const T kx = (T)kfreq(t + m * NT, N);
-        const eidx e = at(m);
-        if (j == 0) {
-          const cpx<T> u1 = ldx(uh + C, e), u2 = ldx(uh + 2 * C, e);
-          v[m] = mul_i(ky * u2 - kz * u1);
-        } else if (j == 1) {
-          const cpx<T> u0 = ldx(uh, e), u2 = ldx(uh + 2 * C, e);
-          v[m] = mul_i(kz * u0 - kx * u2);
-        } else {
-          const cpx<T> u0 = ldx(uh, e), u1 = ldx(uh + C, e);
-          v[m] = mul_i(kx * u1 - ky * u0);
-        }
-      }
-      FFT::run(v, lds, map, t, twr);
-      if (col.valid)
+    for (int m = 0; m < P; ++m) v[m] = mul_i(ky * u2[m] - kz * u1[m]);
+    fft_store(v, 3);
 #pragma unroll
-        for (int m = 0; m < P; ++m) stx(W + (3 + j) * C, at(m), v[m]);
-    }
+    for (int m = 0; m < P; ++m) v[m] = mul_i(kz * u0[m] - kx(m) * u2[m]);
+    fft_store(v, 4);
+#pragma unroll
+    for (int m = 0; m < P; ++m) v[m] = mul_i(kx(m) * u1[m] - ky * u0[m]);
+    fft_store(v, 5);
+    fft_store(u0, 0);
+    fft_store(u1, 1);
+    fft_store(u2, 2);
   }
 }
 
