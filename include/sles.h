@@ -42,13 +42,15 @@ int sles_plan_destroy(sles_plan plan);
 int sles_plan_info(sles_plan plan, int* n, int* nh, int* precision);
 
 /* Kernel variants of the fused step, chosen per plan (by the autotuner,
- * sles/autotune.py, which persists its choices in a JSON table).  Value 0 is
- * the default.  No counterpart in the reference.
+ * sles/autotune.py, which persists its choices in a JSON table).  Defaults:
+ * 0 for the three plan choices, 1 for SLES_TUNE_XCD.  No counterpart in the
+ * reference.
  *   SLES_TUNE_STRIDED_RADIX4  x/y inverse + y forward passes use the radix-4
  *                             line plan (4 points per thread) instead of radix-8
  *   SLES_TUNE_XFWD_RADIX4     same for the x-forward + stage-update pass
  *   SLES_TUNE_Z_RADIX4        same for the fused z kernel
- *   SLES_TUNE_XCD             XCD-aware tile order in the strided passes
+ *   SLES_TUNE_XCD             XCD-aware tile order in the strided passes (fused
+ *                             step and the generic transforms)
  * The radix-4 plans exist for N = 32, 64, 128, 256, 512 (ignored otherwise). */
 enum { SLES_TUNE_STRIDED_RADIX4 = 0, SLES_TUNE_XFWD_RADIX4 = 1, SLES_TUNE_Z_RADIX4 = 2, SLES_TUNE_XCD = 3 };
 int sles_plan_set_tuning(sles_plan plan, int key, int value);
@@ -233,6 +235,15 @@ int sles_adjoint_rhs(sles_plan plan, const sles_step_params* params, const void*
  * lam2 may be NULL (taken as 0).  3 spectral fields each. */
 int sles_etd2_adjoint_combine(sles_plan plan, double dt, double nu, const void* lam1, const void* lam2, void* mu,
                               void* lam_out, void* stream);
+/* One ETD2 backward step in one call (the combination above fused into the
+ * adjoint passes, mu never stored):
+ *   out = e^{hL} lam1 + J_F(y)^T (h (phi1 + phi2)(hL) lam1 - h phi2(hL) lam2)
+ * with h = params->dt, J_F the Jacobian of the nonlinear term (linear = 2
+ * above: forcing, no viscous term) and the cs_grad accumulation of
+ * sles_adjoint_rhs.  lam2 may be NULL (taken as 0); out must not alias the
+ * inputs; `work` as for sles_adjoint_rhs. */
+int sles_etd2_adjoint_rhs(sles_plan plan, const sles_step_params* params, const void* y, const void* lam1,
+                          const void* lam2, void* out, double dcoeff, double* cs_grad, void* work, void* stream);
 /* y = a x + b y over nfields spectral fields (contiguous). */
 int sles_axpby(sles_plan plan, int nfields, double a, const void* x, double b, void* y, void* stream);
 

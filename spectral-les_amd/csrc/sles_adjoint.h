@@ -40,6 +40,44 @@ __device__ __forceinline__ cpx<T> mul_mis(T s, cpx<T> a) {  // (-i s) a
   return {s * a.y, -s * a.x};
 }
 
+// Where the x-inverse and x-forward passes take the adjoint mu from: the
+// array mu, or (ETD2 backward step) mu = w1 lam1 - w0 lam2 formed per mode,
+// with the output out = E lam1 + J^T mu (E = e^{hL}, L = -nu K^2,
+// w1 = h (phi1 + phi2), w0 = h phi2).
+struct AdjSrc {
+  const void* mu;
+  const void* lam1;
+  const void* lam2;  // may be null (treated as zero)
+  double h;
+  int etd2;
+};
+template <typename T>
+struct MuSrc {
+  const cpx<T>* mu;
+  const cpx<T>* l1;
+  const cpx<T>* l2;
+  double h, nu;
+  int etd2;
+  __device__ __forceinline__ void weights(long k2, T& w1, T& w0, T& E) const {
+    const double z = -h * nu * (double)k2;
+    const double p1 = phi1_d(z), p2 = phi2_d(z);
+    w1 = (T)(h * (p1 + p2));
+    w0 = (T)(h * p2);
+    E = (T)exp(z);
+  }
+  // mu of component j at element e (offset within the field); ETD2: weights w1, w0 of that mode
+  template <bool ETD2>
+  __device__ __forceinline__ cpx<T> at(int j, long C, long e, T w1, T w0) const {
+    if constexpr (!ETD2) {
+      return mu[j * C + e];
+    } else {
+      cpx<T> r = w1 * l1[j * C + e];
+      if (l2) r = r - w0 * l2[j * C + e];
+      return r;
+    }
+  }
+};
+
 // acc[0] += sum of part[0..np)  (one block, fixed order)
 static __global__ void k_sum_add(const double* part, int np, double* acc) {
   __shared__ double sh[1024];
@@ -57,9 +95,9 @@ static __global__ void k_sum_add(const double* part, int np, double* acc) {
 // ---------------------------------------------------------------------------
 // 1. x-inverse
 // ---------------------------------------------------------------------------
-template <typename T, int N, class PL>
+template <typename T, int N, class PL, bool ETD2>
 __global__ void __launch_bounds__((StridedShape<T, N, PL>::THREADS))
-    k_adj_xinv(const cpx<T>* __restrict__ Y, const cpx<T>* __restrict__ mu, cpx<T>* __restrict__ A, long C,
+    k_adj_xinv(const cpx<T>* __restrict__ Y, MuSrc<T> mu, cpx<T>* __restrict__ A, long C,
                const cpx<T>* __restrict__ tw, int xcd) {
   using S = StridedShape<T, N, PL>;
   constexpr int P = S::P, NT = S::NT, TK = S::TK, NH = N / 2 + 1;
@@ -67,7 +105,8 @@ __global__ void __launch_bounds__((StridedShape<T, N, PL>::THREADS))
   __shared__ cpx<T> lds[N * TK];
   const Cols<N, TK> col(xcd);
   const int t = threadIdx.x / TK;
-  const TwRegs<T, PL> twr(tw, t);  // read once per block, used by every transform
+  // FP64 radix-8 ETD2 variant: table twiddles (the ETD2 weights need the registers)
+  const auto twr = TwPick<T, PL, !(ETD2 && sizeof(T) == 8 && PL::P >= 8)>::make(tw, t);
   const LdsColumn<TK> map{col.c};
   const int ky = kfreq((int)(col.g / NH), N), kz = (int)(col.g % NH);
   auto at = [&](int m) { return (long)(t + m * NT) * Cols<N, TK>::M + col.g; };
@@ -87,26 +126,43 @@ __global__ void __launch_bounds__((StridedShape<T, N, PL>::THREADS))
 #pragma unroll
     for (int m = 0; m < P; ++m) A[(3 + j) * C + at(m)] = w[m];
   }
-  // g = dealias * Pi mu; s = (K . mu) / K^2
+  // g = dealias * Pi mu; s = (K . mu) / K^2.  With room in the register file
+  // (FP32, or 4 points per thread) the three mu components stay in registers
+  // for the second loop; otherwise they are read again (with the ETD2 weights
+  // of each mode kept).
+  constexpr bool HOLD = sizeof(T) == 4 || P <= 4;
   cpx<T> s[P];
+  cpx<T> mh0[HOLD ? P : 1], mh1[HOLD ? P : 1], mh2[HOLD ? P : 1];
+  T w1[HOLD ? 1 : P], w0[HOLD ? 1 : P];
 #pragma unroll
   for (int m = 0; m < P; ++m) {
     const int kx = kfreq(t + m * NT, N);
     const long k2 = (long)kx * kx + (long)ky * ky + (long)kz * kz;
     const T inv = T(1) / (T)(k2 == 0 ? 1 : k2);
     const long e = at(m);
-    const cpx<T> m0 = mu[e], m1 = mu[C + e], m2 = mu[2 * C + e];
+    T a1 = T(0), a0 = T(0), E;
+    if constexpr (ETD2) mu.weights(k2, a1, a0, E);
+    const cpx<T> m0 = mu.template at<ETD2>(0, C, e, a1, a0), m1 = mu.template at<ETD2>(1, C, e, a1, a0),
+                 m2 = mu.template at<ETD2>(2, C, e, a1, a0);
+    if constexpr (HOLD) {
+      mh0[m] = m0;
+      mh1[m] = m1;
+      mh2[m] = m2;
+    } else {
+      w1[m] = a1;
+      w0[m] = a0;
+    }
     s[m] = inv * (((T)kx * m0 + (T)ky * m1) + (T)kz * m2);
   }
-#pragma unroll 1
-  for (int j = 0; j < 3; ++j) {
+  // component j of g and of -2i Kx g, transformed and stored
+  auto emit = [&](int j, const cpx<T>(&mj)[P]) {
 #pragma unroll
     for (int m = 0; m < P; ++m) {
       const int kx = kfreq(t + m * NT, N);
       const KIndex k = {kx, ky, kz};
       const T keep = dealias_keep(k, N) ? T(1) : T(0);
       const T kj = (T)(j == 0 ? kx : (j == 1 ? ky : kz));
-      v[m] = keep * (mu[j * C + at(m)] - kj * s[m]);
+      v[m] = keep * (mj[m] - kj * s[m]);
       w[m] = mul_mis(T(2) * (T)kx, v[m]);
     }
     FFT::run(v, lds, map, t, twr);
@@ -115,6 +171,20 @@ __global__ void __launch_bounds__((StridedShape<T, N, PL>::THREADS))
     FFT::run(w, lds, map, t, twr);
 #pragma unroll
     for (int m = 0; m < P; ++m) A[(12 + j) * C + at(m)] = w[m];
+  };
+  if constexpr (HOLD) {
+    // straight-line: each held component dies after its use
+    emit(0, mh0);
+    emit(1, mh1);
+    emit(2, mh2);
+  } else {
+#pragma unroll 1
+    for (int j = 0; j < 3; ++j) {
+      cpx<T> mj[P];
+#pragma unroll
+      for (int m = 0; m < P; ++m) mj[m] = mu.template at<ETD2>(j, C, at(m), w1[m], w0[m]);
+      emit(j, mj);
+    }
   }
 }
 
@@ -492,9 +562,9 @@ struct AdjArgs {
   int accumulate;  // out += result instead of out = result
 };
 
-template <typename T, int N, class PL>
+template <typename T, int N, class PL, bool ETD2>
 __global__ void __launch_bounds__((StridedShape<T, N, PL>::THREADS))
-    k_adj_xfwd(const cpx<T>* __restrict__ A, long C, const cpx<T>* __restrict__ mu, cpx<T>* out, AdjArgs a,
+    k_adj_xfwd(const cpx<T>* __restrict__ A, long C, MuSrc<T> mu, cpx<T>* out, AdjArgs a,
                const cpx<T>* __restrict__ tw, int xcd) {
   using S = StridedShape<T, N, PL>;
   constexpr int P = S::P, NT = S::NT, TK = S::TK, NH = N / 2 + 1;
@@ -502,7 +572,8 @@ __global__ void __launch_bounds__((StridedShape<T, N, PL>::THREADS))
   __shared__ cpx<T> lds[N * TK];
   const Cols<N, TK> col(xcd);
   const int t = threadIdx.x / TK;
-  const TwRegs<T, PL> twr(tw, t);  // read once per block, used by every transform
+  // FP64 radix-8 ETD2 variant: table twiddles (the ETD2 weights need the registers)
+  const auto twr = TwPick<T, PL, !(ETD2 && sizeof(T) == 8 && PL::P >= 8)>::make(tw, t);
   const LdsColumn<TK> map{col.c};
   auto at = [&](int m) { return (long)(t + m * NT) * Cols<N, TK>::M + col.g; };
   const int ky = kfreq((int)(col.g / NH), N), kz = (int)(col.g % NH);
@@ -523,22 +594,30 @@ __global__ void __launch_bounds__((StridedShape<T, N, PL>::THREADS))
 #pragma unroll
       for (int m = 0; m < P; ++m) {
         const long e = j * C + at(m);
+        const int kx = kfreq(t + m * NT, N);
+        const long k2 = (long)kx * kx + (long)ky * ky + (long)kz * kz;
         cpx<T> res = sc * (v[m] + r[m]);
-        if (a.linear) {
-          const int kx = kfreq(t + m * NT, N);
-          const long k2 = (long)kx * kx + (long)ky * ky + (long)kz * kz;
-          const double visc = a.linear == 1 ? a.nu * (double)k2 : 0.0;
-          const T phi = (T)((a.forcing > 0.0 && k2 != 0 ? a.forcing : 0.0) - visc);
-          res = res + phi * mu[e];
+        if constexpr (ETD2) {
+          // e^{hL} lam_{n+1} + J^T mu, with the forcing part of the linear term
+          T w1, w0, E;
+          mu.weights(k2, w1, w0, E);
+          if (a.forcing > 0.0 && k2 != 0) res = res + (T)a.forcing * mu.template at<true>(j, C, at(m), w1, w0);
+          out[e] = E * mu.l1[e] + res;
+        } else {
+          if (a.linear) {
+            const double visc = a.linear == 1 ? a.nu * (double)k2 : 0.0;
+            const T phi = (T)((a.forcing > 0.0 && k2 != 0 ? a.forcing : 0.0) - visc);
+            res = res + phi * mu.mu[e];
+          }
+          out[e] = a.accumulate ? out[e] + res : res;
         }
-        out[e] = a.accumulate ? out[e] + res : res;
       }
     }
   }
 }
 
 template <typename T, int N, class PS, class PZ>
-int adjoint_impl(const sles_plan_s* p, const sles_step_params* prm, const void* y, const void* mu, void* out,
+int adjoint_impl(const sles_plan_s* p, const sles_step_params* prm, const void* y, const AdjSrc& src, void* out,
                  int accumulate, int linear, double dcoeff, double* cs_grad, void* work, hipStream_t s) {
   using SS = StridedShape<T, N, PS>;
   constexpr int ZB = (int)(((long)N * N + ZLines<PZ>::LPB - 1) / ZLines<PZ>::LPB);
@@ -550,7 +629,12 @@ int adjoint_impl(const sles_plan_s* p, const sles_step_params* prm, const void* 
   constexpr long NCOL = (long)N * (N / 2 + 1);  // columns of every strided pass
   const dim3 gs((unsigned)((NCOL + SS::TK - 1) / SS::TK)), bs(SS::THREADS);
   const T scale = T(1.0 / ((double)N * N * N));
-  hipLaunchKernelGGL((k_adj_xinv<T, N, PS>), gs, bs, 0, s, (const cpx<T>*)y, (const cpx<T>*)mu, A, C, tw, xcd);
+  const MuSrc<T> ms{(const cpx<T>*)src.mu, (const cpx<T>*)src.lam1, (const cpx<T>*)src.lam2, src.h, prm->nu,
+                    src.etd2};
+  if (src.etd2)
+    hipLaunchKernelGGL((k_adj_xinv<T, N, PS, true>), gs, bs, 0, s, (const cpx<T>*)y, ms, A, C, tw, xcd);
+  else
+    hipLaunchKernelGGL((k_adj_xinv<T, N, PS, false>), gs, bs, 0, s, (const cpx<T>*)y, ms, A, C, tw, xcd);
   hipLaunchKernelGGL((k_adj_yinv<T, N, PS>), gs, bs, 0, s, A, C, tw, xcd);
   double* cp = cs_grad ? part : nullptr;
   switch (prm->les_model) {
@@ -570,13 +654,16 @@ int adjoint_impl(const sles_plan_s* p, const sles_step_params* prm, const void* 
   if (cp) hipLaunchKernelGGL(k_sum_add, dim3(1), dim3(1024), 0, s, (const double*)part, ZB, cs_grad);
   hipLaunchKernelGGL((k_adj_yfwd<T, N, PS>), gs, bs, 0, s, A, C, tw, xcd);
   AdjArgs aa{prm->nu, prm->forcing, linear, accumulate};
-  hipLaunchKernelGGL((k_adj_xfwd<T, N, PS>), gs, bs, 0, s, (const cpx<T>*)A, C, (const cpx<T>*)mu, (cpx<T>*)out, aa,
-                     tw, xcd);
+  if (src.etd2)
+    hipLaunchKernelGGL((k_adj_xfwd<T, N, PS, true>), gs, bs, 0, s, (const cpx<T>*)A, C, ms, (cpx<T>*)out, aa, tw, xcd);
+  else
+    hipLaunchKernelGGL((k_adj_xfwd<T, N, PS, false>), gs, bs, 0, s, (const cpx<T>*)A, C, ms, (cpx<T>*)out, aa, tw,
+                       xcd);
   return check_launch("sles_adjoint_rhs");
 }
 
 template <typename T, int N>
-int adjoint_t(const sles_plan_s* p, const sles_step_params* prm, const void* y, const void* mu, void* out,
+int adjoint_t(const sles_plan_s* p, const sles_step_params* prm, const void* y, const AdjSrc& mu, void* out,
               int accumulate, int linear, double dcoeff, double* cs_grad, void* work, hipStream_t s) {
   using D = LinePlan<N>;
   if constexpr (HasP4<N>::value) {
@@ -589,9 +676,9 @@ int adjoint_t(const sles_plan_s* p, const sles_step_params* prm, const void* y, 
   return adjoint_impl<T, N, D, D>(p, prm, y, mu, out, accumulate, linear, dcoeff, cs_grad, work, s);
 }
 
-int adjoint_f64(const sles_plan_s* p, const sles_step_params* prm, const void* y, const void* mu, void* out,
+int adjoint_f64(const sles_plan_s* p, const sles_step_params* prm, const void* y, const AdjSrc& mu, void* out,
                 int accumulate, int linear, double dcoeff, double* cs_grad, void* work, hipStream_t s);
-int adjoint_f32(const sles_plan_s* p, const sles_step_params* prm, const void* y, const void* mu, void* out,
+int adjoint_f32(const sles_plan_s* p, const sles_step_params* prm, const void* y, const AdjSrc& mu, void* out,
                 int accumulate, int linear, double dcoeff, double* cs_grad, void* work, hipStream_t s);
 
 }  // namespace sles

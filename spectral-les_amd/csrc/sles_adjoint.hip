@@ -10,15 +10,6 @@ __global__ void k_axpby(long n, double a, const cpx<T>* x, double b, cpx<T>* y) 
     y[e] = (T)a * x[e] + (T)b * y[e];
 }
 
-__device__ __forceinline__ double phi1_e(double z) {
-  if (fabs(z) < 1e-2) return 1 + z / 2 * (1 + z / 3 * (1 + z / 4 * (1 + z / 5 * (1 + z / 6))));
-  return expm1(z) / z;
-}
-__device__ __forceinline__ double phi2_e(double z) {
-  if (fabs(z) < 1e-2) return 0.5 * (1 + z / 3 * (1 + z / 4 * (1 + z / 5 * (1 + z / 6 * (1 + z / 7)))));
-  return (expm1(z) - z) / (z * z);
-}
-
 template <typename T>
 __global__ void k_etd2_adj(long nc, int n, int nh, double h, double nu, const cpx<T>* l1, const cpx<T>* l2,
                            cpx<T>* mu, cpx<T>* lo) {
@@ -27,7 +18,7 @@ __global__ void k_etd2_adj(long nc, int n, int nh, double h, double nu, const cp
     const KIndex k = kindex(e % nc, n, nh);
     const long k2 = (long)k.kx * k.kx + (long)k.ky * k.ky + (long)k.kz * k.kz;
     const double z = -h * nu * (double)k2;
-    const double p1 = phi1_e(z), p2 = phi2_e(z);
+    const double p1 = phi1_d(z), p2 = phi2_d(z);
     const cpx<T> a = l1[e];
     const cpx<T> b = l2 ? l2[e] : cpx<T>{T(0), T(0)};
     mu[e] = (T)(h * (p1 + p2)) * a - (T)(h * p2) * b;
@@ -55,8 +46,22 @@ int sles_adjoint_rhs(sles_plan p, const sles_step_params* prm, const void* y, co
   if (out == mu || out == y) return fail("sles_adjoint_rhs: out must not alias y or mu");
   if (!sles_supported_n(p->n)) return fail("sles_adjoint_rhs: unsupported grid size");
   hipStream_t s = (hipStream_t)stream;
-  return p->precision == SLES_FP64 ? adjoint_f64(p, prm, y, mu, out, accumulate, linear, dcoeff, cs_grad, work, s)
-                                   : adjoint_f32(p, prm, y, mu, out, accumulate, linear, dcoeff, cs_grad, work, s);
+  const AdjSrc src{mu, nullptr, nullptr, 0.0, 0};
+  return p->precision == SLES_FP64 ? adjoint_f64(p, prm, y, src, out, accumulate, linear, dcoeff, cs_grad, work, s)
+                                   : adjoint_f32(p, prm, y, src, out, accumulate, linear, dcoeff, cs_grad, work, s);
+}
+
+int sles_etd2_adjoint_rhs(sles_plan p, const sles_step_params* prm, const void* y, const void* lam1,
+                          const void* lam2, void* out, double dcoeff, double* cs_grad, void* work, void* stream) {
+  if (!p || !prm || !y || !lam1 || !out || !work) return fail("sles_etd2_adjoint_rhs: null argument");
+  if (prm->les_model < 0 || prm->les_model > 2) return fail("sles_etd2_adjoint_rhs: les_model must be 0, 1 or 2");
+  if (out == lam1 || out == lam2 || out == y) return fail("sles_etd2_adjoint_rhs: out must not alias an input");
+  if (!(prm->dt > 0.0)) return fail("sles_etd2_adjoint_rhs: dt must be positive");
+  if (!sles_supported_n(p->n)) return fail("sles_etd2_adjoint_rhs: unsupported grid size");
+  hipStream_t s = (hipStream_t)stream;
+  const AdjSrc src{nullptr, lam1, lam2, prm->dt, 1};
+  return p->precision == SLES_FP64 ? adjoint_f64(p, prm, y, src, out, 0, 2, dcoeff, cs_grad, work, s)
+                                   : adjoint_f32(p, prm, y, src, out, 0, 2, dcoeff, cs_grad, work, s);
 }
 
 int sles_etd2_adjoint_combine(sles_plan p, double dt, double nu, const void* lam1, const void* lam2, void* mu,

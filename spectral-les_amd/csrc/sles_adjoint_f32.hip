@@ -3,7 +3,7 @@
 
 namespace sles {
 
-int adjoint_f32(const sles_plan_s* p, const sles_step_params* prm, const void* y, const void* mu, void* out,
+int adjoint_f32(const sles_plan_s* p, const sles_step_params* prm, const void* y, const AdjSrc& mu, void* out,
                 int accumulate, int linear, double dcoeff, double* cs_grad, void* work, hipStream_t s) {
 #define SLES_CASE(NN) \
   case NN:            \

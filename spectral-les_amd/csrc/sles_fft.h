@@ -304,6 +304,19 @@ struct TwRegs {
   }
 };
 
+// Register twiddles when REG, else the table pointer (for kernels that are
+// already near the register limit).
+template <typename T, class PL, bool REG>
+struct TwPick {
+  using type = TwRegs<T, PL>;
+  __device__ __forceinline__ static type make(const cpx<T>* __restrict__ tw, int t) { return type(tw, t); }
+};
+template <typename T, class PL>
+struct TwPick<T, PL, false> {
+  using type = const cpx<T>*;
+  __device__ __forceinline__ static type make(const cpx<T>* __restrict__ tw, int) { return tw; }
+};
+
 // ---------------------------------------------------------------------------
 // One stage: twiddle + butterflies on the P register slots of thread t.
 // ---------------------------------------------------------------------------

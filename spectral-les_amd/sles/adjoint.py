@@ -167,11 +167,13 @@ class DifferentiableLES:
             for n in reversed(range(start, end)):
                 Un = states[n - start]
                 if n >= 1:
+                    # lam_n = E lam_{n+1} + J_F(U_n)^T (w1 lam_{n+1} - w0 lam_{n+2}), one fused call
                     lam_out = torch.empty_like(lam_next)
-                    _lib.call("sles_etd2_adjoint_combine", d.plan.handle, float(self.dt), float(self.nu),
+                    prm = fused.params(d, dt=self.dt)
+                    _lib.call("sles_etd2_adjoint_rhs", d.plan.handle, ctypes.byref(prm), Un.data_ptr(),
                               lam_next.data_ptr(), lam_next2.data_ptr() if lam_next2 is not None else None,
-                              mu.data_ptr(), lam_out.data_ptr(), d.stream)
-                    self.vjp(Un, mu, lam_out, accumulate=True, linear=2)
+                              lam_out.data_ptr(), self.dcoeff(), self.cs_grad.data_ptr() if d.LES_ON else None,
+                              self.adj_work.data_ptr(), d.stream)
                     lam_next2, lam_next = lam_next, lam_out
                 else:
                     # step 0: RK4 bootstrap U_1 = RK4(U_0), plus F(U_0) in the step to U_2

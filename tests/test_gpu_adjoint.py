@@ -169,3 +169,46 @@ def test_etd2_adjoint_vs_fd(les):
     v *= float(U0.abs().max()) * 1e-6 / float(v.abs().max())
     dd = (L(U0 + v, Cs) - L(U0 - v, Cs)) / 2
     assert abs(rdot(U.grad, v) - dd) < 1e-5 * abs(dd)
+
+
+@pytest.mark.parametrize("les", [None, "Smagorinsky", "Vreman"])
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("with_lam2", [True, False])
+def test_fused_etd2_adjoint_step_matches_combine_plus_vjp(les, precision, with_lam2):
+    """sles_etd2_adjoint_rhs == sles_etd2_adjoint_combine followed by
+    sles_adjoint_rhs(accumulate, linear = 2), values and d/dCs."""
+    import ctypes
+
+    from sles import _lib, fused
+
+    N = 16
+    s = _solver(N, les, scheme="ETD2", dt=0.004, nu=0.02, precision=precision, forcing=0.1)
+    s._set_cs(0.15)
+    d = s.d
+    cd = d.cdtype
+    Y = _state(N, 21).to(cd)
+    g = torch.Generator(device="cuda").manual_seed(22)
+
+    def rnd():
+        return torch.complex(torch.randn(Y.shape, generator=g, device="cuda", dtype=torch.float64),
+                             torch.randn(Y.shape, generator=g, device="cuda", dtype=torch.float64)).to(cd)
+
+    lam1, lam2 = rnd(), (rnd() if with_lam2 else None)
+    mu = torch.empty_like(lam1)
+    ref = torch.empty_like(lam1)
+    _lib.call("sles_etd2_adjoint_combine", d.plan.handle, s.dt, s.nu, lam1.data_ptr(),
+              lam2.data_ptr() if with_lam2 else None, mu.data_ptr(), ref.data_ptr(), d.stream)
+    s.cs_grad.zero_()
+    s.vjp(Y, mu, ref, accumulate=True, linear=2)
+    cs_ref = float(s.cs_grad.item())
+    out = torch.empty_like(lam1)
+    s.cs_grad.zero_()
+    prm = fused.params(d, dt=s.dt)
+    _lib.call("sles_etd2_adjoint_rhs", d.plan.handle, ctypes.byref(prm), Y.data_ptr(), lam1.data_ptr(),
+              lam2.data_ptr() if with_lam2 else None, out.data_ptr(), s.dcoeff(),
+              s.cs_grad.data_ptr() if d.LES_ON else None, s.adj_work.data_ptr(), d.stream)
+    torch.cuda.synchronize()
+    tol = 1e-12 if precision == "fp64" else 1e-5
+    assert float((out - ref).abs().max()) <= tol * float(ref.abs().max())
+    if les:
+        assert abs(float(s.cs_grad.item()) - cs_ref) <= tol * abs(cs_ref)
