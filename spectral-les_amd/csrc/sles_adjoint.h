@@ -266,6 +266,30 @@ __global__ void __launch_bounds__((StridedShape<T, N, PL>::THREADS))
 // ---------------------------------------------------------------------------
 // 3. z kernel
 // ---------------------------------------------------------------------------
+// A pair of half-spectra rows A, B (per-thread row pointers) as Z_k = A_k + i B_k,
+// optionally multiplied by i*kz first, Hermitian completion as c2r.
+template <typename T, int N, int P, int NT>
+__device__ __forceinline__ void load_pair_rows(cpx<T> (&v)[P], const cpx<T>* A, bool dzA, const cpx<T>* B, bool dzB,
+                                               int t, bool valid) {
+#pragma unroll
+  for (int m = 0; m < P; ++m) {
+    const int k = t + m * NT;
+    // A, B point at an in-bounds row even for an invalid line (never stored)
+    const int kk = k <= N / 2 ? k : N - k;
+    cpx<T> a = A[kk], b = B[kk];
+    if (dzA) a = mul_i((T)kk * a);
+    if (dzB) b = mul_i((T)kk * b);
+    if (k > N / 2) {
+      a = conj(a);
+      b = conj(b);
+    } else if (k == 0 || k == N / 2) {
+      a.y = T(0);
+      b.y = T(0);
+    }
+    v[m] = {a.x - b.y, a.y + b.x};
+  }
+}
+
 // One c2r operand: (F1 [+ (-2i kz) F2]) / Wz(k), Hermitian completion as c2r.
 template <typename T, int N>
 __device__ __forceinline__ cpx<T> adj_term(const cpx<T>* F1, const cpx<T>* F2, int k) {
@@ -329,7 +353,7 @@ __global__ void __launch_bounds__(ZLines<PL>::THREADS)
   T u0[P], u1[P], u2[P], a[3][3][P];
   cpx<T> v[P];
 #define SLES_INV(PA, DA, PB, DB, RA, RB)                           \
-  load_pair<T, N, P, NT>(v, PA, DA, PB, DB, t, valid);             \
+  load_pair_rows<T, N, P, NT>(v, PA, DA, PB, DB, t, valid);        \
   IFFT::run(v, lds, map, t, twr);                                   \
   _Pragma("unroll") for (int m = 0; m < P; ++m) {                  \
     RA[m] = scale * v[m].x;                                        \

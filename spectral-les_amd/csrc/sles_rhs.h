@@ -38,10 +38,6 @@ namespace sles {
 
 enum { EPI_RK4 = 0, EPI_RK3 = 1, EPI_ETD2 = 2, EPI_RHS = 3, EPI_NONLIN = 4 };
 
-#ifndef SLES_PF
-#define SLES_PF 1
-#endif
-
 // Element index inside one spectral field and the accessors of the strided
 // passes.
 using eidx = long;
@@ -205,33 +201,12 @@ struct ZLines {
   static constexpr int THREADS = LPB * NT;
 };
 
-// Load Z_k = A_k + i B_k (Hermitian completion of two half-spectra, each
-// optionally multiplied by i*kz first; imaginary parts of the kz = 0 and N/2
-// bins dropped as c2r does).
-template <typename T, int N, int P, int NT>
-__device__ __forceinline__ void load_pair(cpx<T> (&v)[P], const cpx<T>* A, bool dzA, const cpx<T>* B, bool dzB,
-                                          int t, bool valid) {
-#pragma unroll
-  for (int m = 0; m < P; ++m) {
-    const int k = t + m * NT;
-    // A, B point at an in-bounds row even for an invalid line (never stored)
-    const int kk = k <= N / 2 ? k : N - k;
-    cpx<T> a = A[kk], b = B[kk];
-    if (dzA) a = mul_i((T)kk * a);
-    if (dzB) b = mul_i((T)kk * b);
-    if (k > N / 2) {
-      a = conj(a);
-      b = conj(b);
-    } else if (k == 0 || k == N / 2) {
-      a.y = T(0);
-      b.y = T(0);
-    }
-    v[m] = {a.x - b.y, a.y + b.x};
-  }
-}
-
-// Raw loads of a pair (bins kk = min(k, N-k)) and the completion step, so
-// that the next pair's loads can be issued before the current transform.
+// Line pairs Z_k = A_k + i B_k of two half-spectra A, B (uniform field bases,
+// line starting at element row0): pair_issue loads bins kk = min(k, N-k),
+// pair_finish multiplies by i*kz where asked, completes the Hermitian half
+// and drops the imaginary parts of the kz = 0 and N/2 bins (as c2r does).
+// Split in two so that the next pair's loads are issued before the current
+// pair's transform.
 template <typename T, int P>
 struct PairRaw {
   cpx<T> a[P], b[P];
@@ -253,28 +228,6 @@ __device__ __forceinline__ void pair_finish(cpx<T> (&v)[P], const PairRaw<T, P>&
     const int k = t + m * NT;
     const int kk = k <= N / 2 ? k : N - k;
     cpx<T> a = r.a[m], b = r.b[m];
-    if (dzA) a = mul_i((T)kk * a);
-    if (dzB) b = mul_i((T)kk * b);
-    if (k > N / 2) {
-      a = conj(a);
-      b = conj(b);
-    } else if (k == 0 || k == N / 2) {
-      a.y = T(0);
-      b.y = T(0);
-    }
-    v[m] = {a.x - b.y, a.y + b.x};
-  }
-}
-
-// Same with uniform field bases A, B and the row's first element `row0`.
-template <typename T, int N, int P, int NT>
-__device__ __forceinline__ void load_pair32(cpx<T> (&v)[P], const cpx<T>* A, bool dzA, const cpx<T>* B, bool dzB,
-                                            unsigned row0, int t) {
-#pragma unroll
-  for (int m = 0; m < P; ++m) {
-    const int k = t + m * NT;
-    const int kk = k <= N / 2 ? k : N - k;
-    cpx<T> a = ld32(A, row0 + kk), b = ld32(B, row0 + kk);
     if (dzA) a = mul_i((T)kk * a);
     if (dzB) b = mul_i((T)kk * b);
     if (k > N / 2) {
@@ -353,7 +306,6 @@ __global__ void __launch_bounds__(ZLines<PL>::THREADS)
   // ---- inverse: 6 complex transforms -> 12 physical fields (scaled 1/N^3)
   T u0[P], u1[P], u2[P], a11[P], a12[P], a13[P], a21[P], a22[P], a23[P], a31[P], a32[P], a33[P];
   cpx<T> v[P];
-#if SLES_PF
   // pair i+1 is loaded while pair i is transformed
   PairRaw<T, P> pr;
   pair_issue<T, N, P, NT>(pr, w[0], w[1], row0, t);
@@ -372,22 +324,6 @@ __global__ void __launch_bounds__(ZLines<PL>::THREADS)
   SLES_INV(false, false, w[7], w[8], a13, a21)
   SLES_INV(false, false, (cpx<T>*)nullptr, (cpx<T>*)nullptr, a22, a23)
 #undef SLES_INV
-#else
-#define SLES_INV(A, DA, B, DB, RA, RB)                             \
-  load_pair32<T, N, P, NT>(v, A, DA, B, DB, row0, t);              \
-  IFFT::run(v, lds, map, t, twr);                                  \
-  _Pragma("unroll") for (int m = 0; m < P; ++m) {                  \
-    RA[m] = scale * v[m].x;                                        \
-    RB[m] = scale * v[m].y;                                        \
-  }
-  SLES_INV(w[0], false, w[1], false, u0, u1)
-  SLES_INV(w[2], false, w[0], true, u2, a31)
-  SLES_INV(w[1], true, w[2], true, a32, a33)
-  SLES_INV(w[3], false, w[4], false, a11, a12)
-  SLES_INV(w[5], false, w[6], false, a13, a21)
-  SLES_INV(w[7], false, w[8], false, a22, a23)
-#undef SLES_INV
-#endif
 
   // ---- pointwise: omega, u x omega, nu_R, tau = nu_R S_ij, max(u)
   T q0[P], q1[P], q2[P], q3[P], q4[P], q5[P], q6[P], q7[P], q8[P];
@@ -496,16 +432,20 @@ __global__ void __launch_bounds__(ZLines<PL>::THREADS)
   for (int i = 0; i < 6; ++i) w[i] = W + i * C;
   T u0[P], u1[P], u2[P], o0[P], o1[P], o2[P];
   cpx<T> v[P];
-#define SLES_INV(A, B, RA, RB)                                     \
-  load_pair32<T, N, P, NT>(v, A, false, B, false, row0, t);       \
+  // pair i+1 is loaded while pair i is transformed
+  PairRaw<T, P> pr;
+  pair_issue<T, N, P, NT>(pr, w[0], w[1], row0, t);
+#define SLES_INV(NA, NB, RA, RB)                                   \
+  pair_finish<T, N, P, NT>(v, pr, false, false, t);                \
+  if (NA) pair_issue<T, N, P, NT>(pr, NA, NB, row0, t);            \
   IFFT::run(v, lds, map, t, twr);                                  \
   _Pragma("unroll") for (int m = 0; m < P; ++m) {                  \
     RA[m] = scale * v[m].x;                                        \
     RB[m] = scale * v[m].y;                                        \
   }
-  SLES_INV(w[0], w[1], u0, u1)
-  SLES_INV(w[2], w[3], u2, o0)
-  SLES_INV(w[4], w[5], o1, o2)
+  SLES_INV(w[2], w[3], u0, u1)
+  SLES_INV(w[4], w[5], u2, o0)
+  SLES_INV((cpx<T>*)nullptr, (cpx<T>*)nullptr, o1, o2)
 #undef SLES_INV
   T umax = -INFINITY;
   T n0[P], n1[P], n2[P];
@@ -724,26 +664,19 @@ __global__ void __launch_bounds__((StridedShape<T, N, PL, 512>::THREADS))
   // a0..a2: S_i = F_x(W_i) (+ 2i Kx F_x(tau_xi) with LES).  Named arrays and
   // a rolled field loop with uniform branches keep them in registers.
   cpx<T> a0[P], a1[P], a2[P], v[P];
-#if SLES_PF
   // next field's loads are issued before the current field's transform
   constexpr int NF = LES ? 6 : 3;
   cpx<T> nx[P];
 #pragma unroll
   for (int m = 0; m < P; ++m) nx[m] = ldx(W, at(m));
-#endif
 #pragma unroll 1
-  for (int f = 0; f < (LES ? 6 : 3); ++f) {
-#if SLES_PF
+  for (int f = 0; f < NF; ++f) {
 #pragma unroll
     for (int m = 0; m < P; ++m) v[m] = nx[m];
     if (f + 1 < NF) {
 #pragma unroll
       for (int m = 0; m < P; ++m) nx[m] = ldx(W + (f + 1) * C, at(m));
     }
-#else
-#pragma unroll
-    for (int m = 0; m < P; ++m) v[m] = ldx(W + f * C, at(m));
-#endif
     FFT::run(v, lds, map, t, twr);
     if (f < 3) {
       if (f == 0) {
