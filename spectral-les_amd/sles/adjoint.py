@@ -106,19 +106,27 @@ class DifferentiableLES:
 
     def run_forward(self, U0, keep=None):
         """Advance nsteps; returns the final state and the checkpoints
-        {step: (U, F_prev)}."""
+        {step: (U, F_prev)}.  For ETD2 the states of the last segment are kept
+        as well (entry "tail": [U_n, ...]): the backward sweep starts there
+        and needs no recomputation for it."""
         d = self.d
         ckpt = {}
         d.U_hat.copy_(U0)
         d.U_hat1.copy_(U0)
         d.firstIteration = True
+        last = ((self.nsteps - 1) // self.K) * self.K
+        tail = []
         for n in range(self.nsteps):
             if n % self.K == 0:
                 ckpt[n] = (d.U_hat.clone(), d.F0.clone() if (self.scheme == "ETD2" and n > 0) else None)
+            if self.scheme == "ETD2" and n >= last:
+                tail.append(ckpt[n][0] if n == last else d.U_hat.clone())
             if self.scheme == "RK4":
                 fused.rk4_step(d, physical=False)
             else:
                 fused.etd2_step(d, physical=False)
+        if tail:
+            ckpt["tail"] = (last, tail)
         return d.U_hat.clone(), ckpt
 
     # -- backward -----------------------------------------------------------------
@@ -149,21 +157,25 @@ class DifferentiableLES:
         # adjoints of the states U_{n+1} and U_{n+2} during the sweep
         lam_next = lam_final.clone()  # lam_{n+1}
         lam_next2 = None  # lam_{n+2}
-        for start in sorted(ckpt, reverse=True):
+        tail_start, tail_states = ckpt.get("tail", (None, None))
+        for start in sorted((k for k in ckpt if k != "tail"), reverse=True):
             end = min(start + self.K, N)
-            # recompute U_start .. U_{end-1}
-            U, Fp = ckpt[start]
-            states = []
-            d.U_hat.copy_(U)
-            d.U_hat1.copy_(U)
-            if start == 0:
-                d.firstIteration = True
+            if start == tail_start:
+                states = tail_states  # kept by the forward pass
             else:
-                d.F0.copy_(Fp)
-                d.firstIteration = False
-            for n in range(start, end):
-                states.append(d.U_hat.clone())
-                fused.etd2_step(d, physical=False)
+                # recompute U_start .. U_{end-1}
+                U, Fp = ckpt[start]
+                states = []
+                d.U_hat.copy_(U)
+                d.U_hat1.copy_(U)
+                if start == 0:
+                    d.firstIteration = True
+                else:
+                    d.F0.copy_(Fp)
+                    d.firstIteration = False
+                for n in range(start, end):
+                    states.append(d.U_hat.clone())
+                    fused.etd2_step(d, physical=False)
             for n in reversed(range(start, end)):
                 Un = states[n - start]
                 if n >= 1:
