@@ -162,9 +162,10 @@ def measured_traffic(kernel, N, precision):
     return None
 
 
-def tgv_gpu(N=64, steps=200, warmup=5):
+def tgv_gpu(N=64, steps=200, warmup=5, graph_steps=10):
     """The CPU-runnable Taylor-Green case: 64^3, nu = 1/1600, dt = 0.01, RK4,
-    no LES, FP64; fused steps timed with events (U and curl written each step)."""
+    no LES, FP64; fused steps timed with events (U and curl written each step),
+    launched one by one and as replays of a HIP graph of graph_steps steps."""
     import numpy as np
     import torch
 
@@ -188,7 +189,30 @@ def tgv_gpu(N=64, steps=200, warmup=5):
     e1.record()
     e1.synchronize()
     ms = e0.elapsed_time(e1) / steps
-    return {"steps_per_s": round(1e3 / ms, 1), "ms_per_step": round(ms, 4), "steps": steps}
+    out = {"steps_per_s": round(1e3 / ms, 1), "ms_per_step": round(ms, 4), "steps": steps}
+    # the same steps captured into one graph (the step is launch-bound at 64^3)
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        fused.rk4_step(d)
+    torch.cuda.current_stream().wait_stream(s)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        for _ in range(graph_steps):
+            fused.rk4_step(d)
+    g.replay()
+    torch.cuda.synchronize()
+    reps = max(1, steps // graph_steps)
+    e0.record()
+    for _ in range(reps):
+        g.replay()
+    e1.record()
+    e1.synchronize()
+    msg = e0.elapsed_time(e1) / (reps * graph_steps)
+    out["graph_steps_per_s"] = round(1e3 / msg, 1)
+    out["graph_ms_per_step"] = round(msg, 4)
+    return out
 
 
 def fft_rates(N, precision, reps=10):
@@ -321,6 +345,7 @@ def main(argv=None):
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-form comparison run")
+    ap.add_argument("--no-512", action="store_true", help="skip the 512^3 FP64 / FP32 step timings in extra")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a plain 1 GiB device copy 5 times (reference kernel for HBM counters)")
     a = ap.parse_args(argv)
@@ -370,6 +395,13 @@ def main(argv=None):
         extra["fp32_speedup_vs_fp64"] = round(v32 / value, 3)
         extra["kernels_fp32"] = roofline_of(r32, N, "fp32")[1]
         log(f"[bench] fp32 {v32:.2f} steps/s")
+    if not a.no_512:
+        for pr in ("fp64", "fp32"):
+            log(f"[bench] 512^3 {pr}")
+            r = run_gpu(512, pr, 10, 2, profile_steps=0)
+            extra[f"grid512_{pr}"] = {"steps_per_s": round(1e4 / r["ms_total"], 3),
+                                       "ms_per_step": round(r["ms_total"] / 10, 3), "steps": 10, "valid": r["ok"]}
+            log(f"[bench] 512^3 {pr}: {extra[f'grid512_{pr}']}")
     extra["fft_3d_batch3"] = {f"{N}_{pr}": fft_rates(N, pr) for pr in ("fp64", "fp32")}
     log(f"[bench] generic 3D FFT: {extra['fft_3d_batch3']}")
     log("[bench] 64^3 Taylor-Green (no LES) FP64")

@@ -53,3 +53,68 @@ def test_fp32_form_vs_fp64(N, les):
         R.solveForNextTimestep(b)
     assert rel(b.U_hat.to(torch.complex128), a.U_hat) < 1e-5
     assert rel(b.curl.double(), a.curl) < 1e-5
+
+
+def _div_and_energy(U_hat, N):
+    """max |K . U_hat| / max(|K| |U_hat|) and the FP64 kinetic energy sum."""
+    dev = U_hat.device
+    k = torch.fft.fftfreq(N, 1.0 / N, dtype=torch.float64, device=dev)
+    kz = torch.arange(N // 2 + 1, dtype=torch.float64, device=dev)
+    U = U_hat.to(torch.complex128)
+    div = k[:, None, None] * U[0] + k[None, :, None] * U[1] + kz[None, None, :] * U[2]
+    kmag = torch.sqrt(k[:, None, None] ** 2 + k[None, :, None] ** 2 + kz[None, None, :] ** 2)
+    scale = float((kmag * U.abs().pow(2).sum(0).sqrt()).max())
+    w = torch.full_like(kmag, 2.0)
+    w[..., 0] = 1.0
+    w[..., -1] = 1.0
+    energy = float((w * U.abs().pow(2).sum(0)).sum())
+    return float(div.abs().max()) / scale, energy
+
+
+@pytest.mark.parametrize("precision,les", [("fp64", "Smagorinsky"), ("fp32", "Vreman")])
+def test_256_steps_solenoidal_and_dissipative(precision, les):
+    """Size-independent properties at the benchmark size: after each fused RK4
+    step (device CFL dt) the velocity stays divergence-free (projection) and
+    the kinetic energy decreases (nu > 0, eddy viscosity >= 0, no forcing)."""
+    from sles import fused
+
+    N = 256
+    d = _domain(N, precision, True, les)
+    div0, e_prev = _div_and_energy(d.U_hat, N)
+    tol = 1e-12 if precision == "fp64" else 1e-5
+    for _ in range(4):
+        fused.rk4_step(d, dt_device=True, cfl=0.5)
+        div, e = _div_and_energy(d.U_hat, N)
+        assert div < tol, div
+        assert e < e_prev
+        e_prev = e
+    assert float(d._scalars[1]) >= 1e-6  # the step was taken
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_512_fft_round_trip(precision):
+    """Generic 3D r2c then c2r at 512^3 returns the input (FP64 1e-13, FP32 5e-6
+    relative), and the r2c of a real field is Hermitian on the kz = 0 plane
+    (to the same tolerance)."""
+    from sles import _lib
+
+    N = 512
+    prec = _lib.SLES_FP64 if precision == "fp64" else _lib.SLES_FP32
+    rdt = torch.float64 if precision == "fp64" else torch.float32
+    cdt = torch.complex128 if precision == "fp64" else torch.complex64
+    plan = _lib.Plan(N, prec, torch.device("cuda"))
+    g = torch.Generator(device="cuda").manual_seed(5)
+    u = torch.randn(1, N, N, N, generator=g, device="cuda", dtype=rdt)
+    U = torch.empty(1, N, N, N // 2 + 1, dtype=cdt, device="cuda")
+    s = torch.cuda.current_stream().cuda_stream
+    _lib.call("sles_fft_r2c", plan.handle, u.data_ptr(), U.data_ptr(), 1, 1.0, s)
+    plane = U[0, :, :, 0]
+    neg = (-torch.arange(N, device="cuda")) % N
+    herm = float((plane - plane[neg][:, neg].conj()).abs().max() / plane.abs().max())
+    v = torch.empty_like(u)
+    _lib.call("sles_fft_c2r", plan.handle, U.data_ptr(), v.data_ptr(), U.data_ptr(), 1, 1.0 / N ** 3, s)
+    torch.cuda.synchronize()
+    err = float((v - u).abs().max() / u.abs().max())
+    tol = 1e-13 if precision == "fp64" else 5e-6
+    assert err < tol, err
+    assert herm < tol, herm
