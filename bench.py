@@ -402,7 +402,8 @@ def main(argv=None):
             extra[f"grid512_{pr}"] = {"steps_per_s": round(1e4 / r["ms_total"], 3),
                                        "ms_per_step": round(r["ms_total"] / 10, 3), "steps": 10, "valid": r["ok"]}
             log(f"[bench] 512^3 {pr}: {extra[f'grid512_{pr}']}")
-    extra["fft_3d_batch3"] = {f"{N}_{pr}": fft_rates(N, pr) for pr in ("fp64", "fp32")}
+    sizes = (N,) if a.no_512 else (N, 512)
+    extra["fft_3d_batch3"] = {f"{n}_{pr}": fft_rates(n, pr) for n in sizes for pr in ("fp64", "fp32")}
     log(f"[bench] generic 3D FFT: {extra['fft_3d_batch3']}")
     log("[bench] 64^3 Taylor-Green (no LES) FP64")
     extra["taylor_green_64_fp64"] = tgv_gpu()

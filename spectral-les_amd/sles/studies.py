@@ -3,8 +3,10 @@
 precision_study  FP32 form vs the FP64 form on the same seeded field and the
                  same dt sequence: velocity rel-L2, max relative E(k) error,
                  steps/s of each form.
-ensemble_run     many members of different grid sizes advanced together, one
+ensemble         many members of different grid sizes advanced together, one
                  captured HIP graph per ensemble step (see ensemble.py).
+ensemble-adjoint d(loss)/dCs of every member through the hand-written adjoint.
+differentiable   Vreman ETD2 gradients against finite differences.
 
     python -m sles.studies precision --n 256 --steps 100 500 --out result.json
 """
@@ -165,6 +167,15 @@ def differentiable_study(N=256, N_target=512, steps=50, target_steps=50, dt=2e-3
             return float(spectrum_loss(solver(U0, torch.tensor(cv, dtype=torch.float64)), target, N, kmax=40))
 
     fd = (L(cs0 + fd_h) - L(cs0 - fd_h)) / (2 * fd_h)
+
+    # initial-field gradient along the radial direction U0: d/da L((1 + a) U0) at a = 0
+    def LU(a_):
+        with torch.no_grad():
+            return float(spectrum_loss(solver((1.0 + a_) * U0, torch.tensor(cs0, dtype=torch.float64)), target, N,
+                                       kmax=40))
+
+    fdU = (LU(fd_h) - LU(-fd_h)) / (2 * fd_h)
+    adjU = float((U.grad.real.double() * U0.real.double() + U.grad.imag.double() * U0.imag.double()).sum())
     res = {
         "N": N, "steps": steps, "dt": dt, "scheme": "ETD2", "les_model": "Vreman", "precision": "fp32", "urms": urms,
         "c_vreman": c_vreman, "Cs": cs0, "checkpoint_every": checkpoint_every,
@@ -172,12 +183,78 @@ def differentiable_study(N=256, N_target=512, steps=50, target_steps=50, dt=2e-3
         "adjoint_vs_fd_rel_diff": abs(float(c.grad) - fd) / max(abs(fd), 1e-300),
         "dloss_dc_vreman_adjoint": float(c.grad) / (5.0 * cs0),
         "grad_U0_norm": float(U.grad.abs().pow(2).sum().sqrt()),
+        "dloss_dalpha_U0_adjoint": adjU, "dloss_dalpha_U0_central_fd_h1e-3": fdU,
+        "U0_direction_rel_diff": abs(adjU - fdU) / max(abs(fdU), 1e-300),
         "max_u_initial": umax0,
         "forward_s": t_fwd, "backward_s": t_bwd,
         "backward_over_forward": t_bwd / t_fwd,
     }
     print(json.dumps(res), flush=True)
     return res
+
+
+def ensemble_adjoint(groups=((64, 8), (96, 8), (128, 8)), steps=200, checkpoint_every=10, nu=1e-4, seed=0,
+                     cs_range=(0.10, 0.20), fd_check=True):
+    """d(loss)/dCs for every member of the ensemble after `steps` RK4 steps
+    (Smagorinsky, FP32 form, hand-written adjoint with checkpointing).  Member
+    i: its own seed, Cs ~ U[cs_range], fixed dt from the CFL rule (0.3) on its
+    initial field.  Loss_i = sum_{k=1}^{N/3} (log E_T(k) - log E_0(k))^2, the
+    squared log-change of the member's spectrum over the window.  Returns the
+    per-member gradients, a central-difference check (h = 1e-3) on the first
+    member, and the forward + adjoint throughput over all members."""
+    import numpy as np
+
+    from .adjoint import DifferentiableLES, shell_spectrum
+    from .SM_solver import compute_dt, find_max_velocity
+
+    rng = np.random.default_rng(seed)
+    members = []
+    i = 0
+    for N, count in groups:
+        for _ in range(count):
+            Cs = float(rng.uniform(*cs_range))
+            s = DifferentiableLES(N=N, nu=nu, dt=1e-3, nsteps=steps, les_model="Smagorinsky", scheme="RK4",
+                                  precision="fp32", checkpoint_every=checkpoint_every)
+            U0 = isotropic_spectrum(N, k0=4.0, seed=seed * 1000 + i, urms=1.0, dtype=torch.complex64)
+            set_spectral_state(s.d, U0)
+            s.dt = compute_dt(s.d.U, find_max_velocity(s.d.U), nu, dx=s.d.dx, CFL=0.3)
+            s.d.dt = s.dt
+            kmax = N // 3
+            E0 = shell_spectrum(U0, N, kmax).detach()
+            members.append((s, U0, Cs, kmax, E0))
+            i += 1
+
+    def loss_of(s, U0, cs, kmax, E0):
+        E = shell_spectrum(s(U0, cs), s.N, kmax)
+        return ((torch.log(E) - torch.log(E0)) ** 2).sum()
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    grads = []
+    for s, U0, Cs, kmax, E0 in members:
+        c = torch.tensor(Cs, dtype=torch.float64, device="cuda", requires_grad=True)
+        L_ = loss_of(s, U0, c, kmax, E0)
+        L_.backward()
+        grads.append({"N": s.N, "Cs": Cs, "dt": s.dt, "loss": float(L_.detach()), "dloss_dCs": float(c.grad)})
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    points = sum(s.N ** 3 for s, *_ in members)
+    out = {"members": grads, "steps": steps, "checkpoint_every": checkpoint_every,
+           "forward_plus_adjoint_s": elapsed,
+           "grid_points_x_steps_per_s_forward_plus_adjoint": points * steps / elapsed}
+    if fd_check:
+        s, U0, Cs, kmax, E0 = members[0]
+        h = 1e-3
+
+        def L(cv):
+            with torch.no_grad():
+                return float(loss_of(s, U0, torch.tensor(cv, dtype=torch.float64), kmax, E0))
+
+        fd = (L(Cs + h) - L(Cs - h)) / (2 * h)
+        out["fd_check_member0"] = {"adjoint": grads[0]["dloss_dCs"], "central_fd_h1e-3": fd,
+                                   "rel_diff": abs(grads[0]["dloss_dCs"] - fd) / max(abs(fd), 1e-300)}
+    print(json.dumps({k: v for k, v in out.items() if k != "members"}), flush=True)
+    return out
 
 
 def main(argv=None):
@@ -196,9 +273,18 @@ def main(argv=None):
     r = sub.add_parser("ensemble")
     r.add_argument("--steps", type=int, default=200)
     r.add_argument("--out", default=None)
+    ra = sub.add_parser("ensemble-adjoint")
+    ra.add_argument("--steps", type=int, default=200)
+    ra.add_argument("--out", default=None)
     a = ap.parse_args(argv)
     if a.cmd == "differentiable":
         res = differentiable_study(a.n, a.n_target, a.steps)
+        if a.out:
+            with open(a.out, "w") as f:
+                json.dump(res, f, indent=1)
+        return
+    if a.cmd == "ensemble-adjoint":
+        res = ensemble_adjoint(steps=a.steps)
         if a.out:
             with open(a.out, "w") as f:
                 json.dump(res, f, indent=1)
