@@ -217,6 +217,14 @@ int sles_profile_read(double* ms, long* count, int nkinds);
 int sles_step(sles_plan plan, int scheme, const sles_step_params* params, void* u_hat, void* u_hat0, void* u_hat1,
               void* aux, void* u_out, void* curl_out, double* scalars, void* work, void* stream);
 
+/* One fused RK4 step (as sles_step with SLES_SCHEME_RK4, host dt, no physical
+ * output) that also writes the stage states Y_1, Y_2, Y_3 (the arguments of
+ * the second to fourth RHS evaluations, 3 spectral fields each) to stages
+ * (9 fields).  u_hat1 must equal u_hat on entry, as for sles_step.  No
+ * counterpart in the reference: the adjoint's recomputation of a step. */
+int sles_rk4_stage_states(sles_plan plan, const sles_step_params* params, void* u_hat, void* u_hat0, void* u_hat1,
+                          void* stages, void* work, void* stream);
+
 /* ---- hand-written adjoint (no counterpart in the reference) ----
  * out (=, or += when accumulate) J(y)^T mu: the vector-Jacobian product of
  * computeRHS (src/RK4_method.py:38-71, with the closure of les_model) at the
@@ -246,6 +254,9 @@ int sles_etd2_adjoint_rhs(sles_plan plan, const sles_step_params* params, const 
                           const void* lam2, void* out, double dcoeff, double* cs_grad, void* work, void* stream);
 /* y = a x + b y over nfields spectral fields (contiguous). */
 int sles_axpby(sles_plan plan, int nfields, double a, const void* x, double b, void* y, void* stream);
+/* out = a x + b y over nfields spectral fields (out may be x or y). */
+int sles_lincomb(sles_plan plan, int nfields, double a, const void* x, double b, const void* y, void* out,
+                 void* stream);
 
 #ifdef __cplusplus
 }

@@ -4,10 +4,11 @@
 
 namespace sles {
 
+// out = a x + b y (out may be y)
 template <typename T>
-__global__ void k_axpby(long n, double a, const cpx<T>* x, double b, cpx<T>* y) {
+__global__ void k_axpby(long n, double a, const cpx<T>* x, double b, const cpx<T>* y, cpx<T>* out) {
   for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x)
-    y[e] = (T)a * x[e] + (T)b * y[e];
+    out[e] = (T)a * x[e] + (T)b * y[e];
 }
 
 template <typename T>
@@ -81,18 +82,28 @@ int sles_etd2_adjoint_combine(sles_plan p, double dt, double nu, const void* lam
   return check_launch("sles_etd2_adjoint_combine");
 }
 
-int sles_axpby(sles_plan p, int nfields, double a, const void* x, double b, void* y, void* stream) {
-  if (!p || !x || !y || nfields < 1) return fail("sles_axpby: bad argument");
+static int axpby_launch(sles_plan p, int nfields, double a, const void* x, double b, const void* y, void* out,
+                        void* stream, const char* what) {
+  if (!p || !x || !y || !out || nfields < 1) return fail(std::string(what) + ": bad argument");
   const long n = (long)nfields * ncplx(p);
   long blocks = (n + 255) / 256;
   if (blocks > 65536) blocks = 65536;
   if (p->precision == SLES_FP64)
     hipLaunchKernelGGL(k_axpby<double>, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, n, a,
-                       (const cpx<double>*)x, b, (cpx<double>*)y);
+                       (const cpx<double>*)x, b, (const cpx<double>*)y, (cpx<double>*)out);
   else
     hipLaunchKernelGGL(k_axpby<float>, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, n, a,
-                       (const cpx<float>*)x, b, (cpx<float>*)y);
-  return check_launch("sles_axpby");
+                       (const cpx<float>*)x, b, (const cpx<float>*)y, (cpx<float>*)out);
+  return check_launch(what);
+}
+
+int sles_axpby(sles_plan p, int nfields, double a, const void* x, double b, void* y, void* stream) {
+  return axpby_launch(p, nfields, a, x, b, y, y, stream, "sles_axpby");
+}
+
+int sles_lincomb(sles_plan p, int nfields, double a, const void* x, double b, const void* y, void* out,
+                 void* stream) {
+  return axpby_launch(p, nfields, a, x, b, y, out, stream, "sles_lincomb");
 }
 
 }  // extern "C"

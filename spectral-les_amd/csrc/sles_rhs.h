@@ -52,6 +52,7 @@ struct StepArgs {
   double nu, forcing;
   double ca, cb;  // RK4: a[s], b[s]
   int scheme, stage;
+  void* stages;  // RK4: if set, stage s < 3 also writes its new state Y_{s+1} to stages + 3 s C
 };
 
 // ---------------------------------------------------------------------------
@@ -608,12 +609,16 @@ __device__ __forceinline__ void epilogue(cpx<T> (&s)[3], eidx e, int kx, int ky,
       cpx<T>* const u0i = uh0 + i * C;
       cpx<T>* const u1i = uh1 + i * C;
       if (a.stage == 0) {
+        const cpx<T> y = u[i] + cb * d[i];
         stx(u0i, e, u[i]);
         stx(u1i, e, u[i] + ca * d[i]);
-        stx(ui, e, u[i] + cb * d[i]);
+        stx(ui, e, y);
+        if (a.stages) stx((cpx<T>*)a.stages + i * C, e, y);
       } else if (a.stage < 3) {
+        const cpx<T> y = ldx(u0i, e) + cb * d[i];
         stx(u1i, e, ldx(u1i, e) + ca * d[i]);
-        stx(ui, e, ldx(u0i, e) + cb * d[i]);
+        stx(ui, e, y);
+        if (a.stages) stx((cpx<T>*)a.stages + (3 * a.stage + i) * C, e, y);
       } else {
         const cpx<T> fin = ldx(u1i, e) + ca * d[i];
         stx(u1i, e, fin);
@@ -849,7 +854,7 @@ struct Fused {
 
 template <typename T, int N, class PS, class PX, class PZ>
 int step_impl(const sles_plan_s* p, int scheme, const sles_step_params* prm, void* uh_, void* uh0_, void* uh1_,
-              void* aux_, void* u_out, void* curl_out, double* scalars, void* work, hipStream_t s) {
+              void* aux_, void* u_out, void* curl_out, double* scalars, void* work, hipStream_t s, void* stages) {
   using F = Fused<T, N, PS, PX, PZ>;
   cpx<T>* uh = (cpx<T>*)uh_;
   cpx<T>* uh0 = (cpx<T>*)uh0_;
@@ -866,6 +871,7 @@ int step_impl(const sles_plan_s* p, int scheme, const sles_step_params* prm, voi
   a.dt_dev = ddt ? scalars + 1 : nullptr;
   a.nu = prm->nu;
   a.forcing = prm->forcing;
+  a.stages = scheme == SLES_SCHEME_RK4 ? stages : nullptr;
   const T coeff = (T)prm->sgs_coeff;
   int nst = 1;
   int epi = EPI_RHS;
@@ -907,14 +913,14 @@ struct HasP4 {
 
 template <typename T, int N>
 int step_t(const sles_plan_s* p, int scheme, const sles_step_params* prm, void* uh, void* uh0, void* uh1, void* aux,
-           void* u_out, void* curl_out, double* scalars, void* work, hipStream_t s) {
+           void* u_out, void* curl_out, double* scalars, void* work, hipStream_t s, void* stages) {
   using D = LinePlan<N>;
   if constexpr (HasP4<N>::value) {
     using Q = LinePlan4<N>;
     const int sel = (p->tune_strided ? 1 : 0) | (p->tune_xfwd ? 2 : 0) | (p->tune_z ? 4 : 0);
 #define SLES_SEL(K, A, B, Cz) \
   case K:                     \
-    return step_impl<T, N, A, B, Cz>(p, scheme, prm, uh, uh0, uh1, aux, u_out, curl_out, scalars, work, s);
+    return step_impl<T, N, A, B, Cz>(p, scheme, prm, uh, uh0, uh1, aux, u_out, curl_out, scalars, work, s, stages);
     switch (sel) {
       SLES_SEL(0, D, D, D)
       SLES_SEL(1, Q, D, D)
@@ -927,13 +933,13 @@ int step_t(const sles_plan_s* p, int scheme, const sles_step_params* prm, void* 
     }
 #undef SLES_SEL
   }
-  return step_impl<T, N, D, D, D>(p, scheme, prm, uh, uh0, uh1, aux, u_out, curl_out, scalars, work, s);
+  return step_impl<T, N, D, D, D>(p, scheme, prm, uh, uh0, uh1, aux, u_out, curl_out, scalars, work, s, stages);
 }
 
 // Per-precision instantiation (sles_rhs_f64.hip / sles_rhs_f32.hip).
 int step_f64(const sles_plan_s* p, int scheme, const sles_step_params* prm, void* uh, void* uh0, void* uh1,
-             void* aux, void* u_out, void* curl_out, double* scalars, void* work, hipStream_t s);
+             void* aux, void* u_out, void* curl_out, double* scalars, void* work, hipStream_t s, void* stages);
 int step_f32(const sles_plan_s* p, int scheme, const sles_step_params* prm, void* uh, void* uh0, void* uh1,
-             void* aux, void* u_out, void* curl_out, double* scalars, void* work, hipStream_t s);
+             void* aux, void* u_out, void* curl_out, double* scalars, void* work, hipStream_t s, void* stages);
 
 }  // namespace sles

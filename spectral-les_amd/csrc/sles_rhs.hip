@@ -34,8 +34,23 @@ int sles_step(sles_plan p, int scheme, const sles_step_params* prm, void* u_hat,
   hipStream_t s = (hipStream_t)stream;
   if (!sles_supported_n(p->n)) return fail("sles_step: unsupported grid size");
   return p->precision == SLES_FP64
-             ? step_f64(p, scheme, prm, u_hat, u_hat0, u_hat1, aux, u_out, curl_out, scalars, work, s)
-             : step_f32(p, scheme, prm, u_hat, u_hat0, u_hat1, aux, u_out, curl_out, scalars, work, s);
+             ? step_f64(p, scheme, prm, u_hat, u_hat0, u_hat1, aux, u_out, curl_out, scalars, work, s, nullptr)
+             : step_f32(p, scheme, prm, u_hat, u_hat0, u_hat1, aux, u_out, curl_out, scalars, work, s, nullptr);
+}
+
+int sles_rk4_stage_states(sles_plan p, const sles_step_params* prm, void* u_hat, void* u_hat0, void* u_hat1,
+                          void* stages, void* work, void* stream) {
+  if (!p || !prm || !u_hat || !u_hat0 || !u_hat1 || !stages || !work)
+    return fail("sles_rk4_stage_states: null argument");
+  if (prm->les_model < 0 || prm->les_model > 2) return fail("sles_rk4_stage_states: les_model must be 0, 1 or 2");
+  if (prm->dt_device) return fail("sles_rk4_stage_states: needs a host dt (dt_device = 0)");
+  if (!sles_supported_n(p->n)) return fail("sles_rk4_stage_states: unsupported grid size");
+  hipStream_t s = (hipStream_t)stream;
+  return p->precision == SLES_FP64
+             ? step_f64(p, SLES_SCHEME_RK4, prm, u_hat, u_hat0, u_hat1, nullptr, nullptr, nullptr, nullptr, work, s,
+                        stages)
+             : step_f32(p, SLES_SCHEME_RK4, prm, u_hat, u_hat0, u_hat1, nullptr, nullptr, nullptr, nullptr, work, s,
+                        stages);
 }
 
 int sles_profile_enable(int on) {

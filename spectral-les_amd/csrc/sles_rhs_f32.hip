@@ -4,10 +4,10 @@
 namespace sles {
 
 int step_f32(const sles_plan_s* p, int scheme, const sles_step_params* prm, void* uh, void* uh0, void* uh1,
-             void* aux, void* u_out, void* curl_out, double* scalars, void* work, hipStream_t s) {
+             void* aux, void* u_out, void* curl_out, double* scalars, void* work, hipStream_t s, void* stages) {
 #define SLES_CASE(NN) \
   case NN:            \
-    return step_t<float, NN>(p, scheme, prm, uh, uh0, uh1, aux, u_out, curl_out, scalars, work, s);
+    return step_t<float, NN>(p, scheme, prm, uh, uh0, uh1, aux, u_out, curl_out, scalars, work, s, stages);
   switch (p->n) {
     SLES_FOR_EACH_N(SLES_CASE)
     default:

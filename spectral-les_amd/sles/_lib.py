@@ -59,11 +59,13 @@ _PROTOS = {
     "sles_adjoint_work_bytes": (_size, [_vp, _int]),
     "sles_adjoint_rhs": (_int, [_vp, _vp, _vp, _vp, _vp, _int, _int, _dbl, _vp, _vp, _vp]),
     "sles_axpby": (_int, [_vp, _int, _dbl, _vp, _dbl, _vp, _vp]),
+    "sles_lincomb": (_int, [_vp, _int, _dbl, _vp, _dbl, _vp, _vp, _vp]),
     "sles_etd2_adjoint_combine": (_int, [_vp, _dbl, _dbl, _vp, _vp, _vp, _vp, _vp]),
     "sles_etd2_adjoint_rhs": (_int, [_vp, _vp, _vp, _vp, _vp, _vp, _dbl, _vp, _vp, _vp]),
     "sles_profile_enable": (_int, [_int]),
     "sles_profile_read": (_int, [_vp, _vp, _int]),
     "sles_step": (_int, [_vp, _int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "sles_rk4_stage_states": (_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
 }
 
 OPTIONAL = set()  # symbols a given build may lack (none at present)

@@ -91,18 +91,17 @@ class DifferentiableLES:
 
     # -- forward ------------------------------------------------------------------
     def _rk4_stages(self, U):
-        """Stage states Y_0..Y_3 of one RK4 step from U and the new state."""
-        Ys = [U]
-        k = torch.empty_like(U)
-        new = U.clone()
-        for s in range(4):
-            self.rhs(Ys[s], k)
-            self.axpby(self.dt * RK4_A[s], k, 1.0, new)
-            if s < 3:
-                Y = U.clone()
-                self.axpby(self.dt * RK4_B[s], k, 1.0, Y)
-                Ys.append(Y)
-        return Ys, new
+        """Stage states Y_0..Y_3 of one RK4 step from U and the new state: one
+        fused RK4 step (the forward pass's own kernels, so the recomputed
+        states equal the forward ones bit for bit) that also writes Y_1..Y_3."""
+        d = self.d
+        d.U_hat.copy_(U)
+        d.U_hat1.copy_(U)
+        st = torch.empty((9,) + tuple(U.shape[1:]), dtype=U.dtype, device=U.device)
+        prm = fused.params(d, dt=self.dt)
+        _lib.call("sles_rk4_stage_states", d.plan.handle, ctypes.byref(prm), d.U_hat.data_ptr(),
+                  d.U_hat0.data_ptr(), d.U_hat1.data_ptr(), st.data_ptr(), fused.work(d).data_ptr(), d.stream)
+        return [U, st[0:3], st[3:6], st[6:9]], d.U_hat.clone()
 
     def run_forward(self, U0, keep=None):
         """Advance nsteps; returns the final state and the checkpoints
@@ -201,17 +200,20 @@ class DifferentiableLES:
                     lam_next2, lam_next = lam_next, lam0
         return lam_next
 
+    def lincomb(self, a, x, b, y, out):
+        _lib.call("sles_lincomb", self.d.plan.handle, 3, float(a), x.data_ptr(), float(b), y.data_ptr(),
+                  out.data_ptr(), self.d.stream)
+
     def _rk4_step_adjoint(self, Ys, lam):
         """lam_n from lam_{n+1} for one RK4 step with stage states Ys."""
         lY = torch.empty_like(lam)
-        mu = lam.clone()
-        self.axpby(0.0, lam, self.dt * RK4_A[3], mu)  # mu = dt a_3 lam
+        mu = torch.empty_like(lam)
+        acc = torch.empty_like(lam)
+        self.lincomb(self.dt * RK4_A[3], lam, 0.0, lam, mu)  # mu = dt a_3 lam
         self.vjp(Ys[3], mu, lY)  # lY = J_3^T mu
-        acc = lam.clone()
-        self.axpby(1.0, lY, 1.0, acc)
+        self.lincomb(1.0, lam, 1.0, lY, acc)  # acc = lam + lY_3
         for s in (2, 1, 0):
-            mu.copy_(lam)
-            self.axpby(self.dt * RK4_B[s], lY, self.dt * RK4_A[s], mu)  # mu = dt b_s lY + dt a_s lam
+            self.lincomb(self.dt * RK4_B[s], lY, self.dt * RK4_A[s], lam, mu)  # mu = dt b_s lY + dt a_s lam
             self.vjp(Ys[s], mu, lY)
             self.axpby(1.0, lY, 1.0, acc)
         return acc

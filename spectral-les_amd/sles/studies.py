@@ -193,6 +193,39 @@ def differentiable_study(N=256, N_target=512, steps=50, target_steps=50, dt=2e-3
     return res
 
 
+def adjoint_cost(N=256, scheme="RK4", les="Smagorinsky", precision="fp32", steps=20, checkpoint_every=10,
+                 dt=1e-3, Cs=0.17, urms=0.25, seed=4):
+    """Backward / forward time of a DifferentiableLES window (loss = sum of
+    E(k), k <= N/3, so the adjoint seed is cheap): the cost of the adjoint step
+    relative to the forward step, with checkpointing."""
+    from .adjoint import DifferentiableLES, shell_spectrum
+
+    s = DifferentiableLES(N=N, nu=1e-4, dt=dt, nsteps=steps, les_model=les, scheme=scheme, precision=precision,
+                          checkpoint_every=checkpoint_every)
+    cd = torch.complex64 if precision == "fp32" else torch.complex128
+    U0 = isotropic_spectrum(N, seed=seed, urms=urms, dtype=cd)
+
+    def once():
+        U = U0.clone().requires_grad_(True)
+        c = torch.tensor(Cs, dtype=torch.float64, device="cuda", requires_grad=True)
+        torch.cuda.synchronize()
+        a = time.perf_counter()
+        out = s(U, c)
+        torch.cuda.synchronize()
+        b = time.perf_counter()
+        shell_spectrum(out, N, N // 3).sum().backward()
+        torch.cuda.synchronize()
+        e = time.perf_counter()
+        return b - a, e - b
+
+    once()
+    f, bwd = once()
+    res = {"N": N, "scheme": scheme, "les_model": les, "precision": precision, "steps": steps,
+           "checkpoint_every": checkpoint_every, "forward_s": f, "backward_s": bwd, "backward_over_forward": bwd / f}
+    print(json.dumps(res), flush=True)
+    return res
+
+
 def ensemble_adjoint(groups=((64, 8), (96, 8), (128, 8)), steps=200, checkpoint_every=10, nu=1e-4, seed=0,
                      cs_range=(0.10, 0.20), fd_check=True):
     """d(loss)/dCs for every member of the ensemble after `steps` RK4 steps
@@ -275,6 +308,12 @@ def main(argv=None):
     r = sub.add_parser("ensemble")
     r.add_argument("--steps", type=int, default=200)
     r.add_argument("--out", default=None)
+    rc = sub.add_parser("adjoint-cost")
+    rc.add_argument("--n", type=int, default=256)
+    rc.add_argument("--scheme", default="RK4")
+    rc.add_argument("--les", default="Smagorinsky")
+    rc.add_argument("--precision", default="fp32")
+    rc.add_argument("--steps", type=int, default=20)
     ra = sub.add_parser("ensemble-adjoint")
     ra.add_argument("--steps", type=int, default=200)
     ra.add_argument("--out", default=None)
@@ -284,6 +323,9 @@ def main(argv=None):
         if a.out:
             with open(a.out, "w") as f:
                 json.dump(res, f, indent=1)
+        return
+    if a.cmd == "adjoint-cost":
+        adjoint_cost(a.n, a.scheme, a.les, a.precision, a.steps)
         return
     if a.cmd == "ensemble-adjoint":
         res = ensemble_adjoint(steps=a.steps)

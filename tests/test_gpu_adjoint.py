@@ -212,3 +212,30 @@ def test_fused_etd2_adjoint_step_matches_combine_plus_vjp(les, precision, with_l
     assert float((out - ref).abs().max()) <= tol * float(ref.abs().max())
     if les:
         assert abs(float(s.cs_grad.item()) - cs_ref) <= tol * abs(cs_ref)
+
+
+@pytest.mark.parametrize("les", [None, "Vreman"])
+def test_rk4_stage_states(les):
+    """sles_rk4_stage_states: the stage states equal U + dt b_s k_s built from
+    the RHS entry point, and the final state equals a plain fused RK4 step
+    bit for bit."""
+    from sles import fused
+
+    N = 16
+    s = _solver(N, les, dt=0.01, nu=0.02)
+    s._set_cs(0.15)
+    U = _state(N, 31)
+    Ys, new = s._rk4_stages(U)
+    # reference stages from the RHS
+    k = torch.empty_like(U)
+    ref = [U]
+    for st in range(3):
+        s.rhs(ref[st], k)
+        ref.append(U + s.dt * (0.5, 0.5, 1.0)[st] * k)
+    for a, b in zip(Ys[1:], ref[1:]):
+        assert float((a - b).abs().max()) < 1e-12 * float(b.abs().max())
+    d = s.d
+    d.U_hat.copy_(U)
+    d.U_hat1.copy_(U)
+    fused.rk4_step(d, dt=s.dt, physical=False)
+    assert torch.equal(d.U_hat, new)
