@@ -79,7 +79,7 @@ def xfwd_mean_fields(les):
     return sum(per_stage) / 4.0
 
 
-def survey_step_bytes(N, precision, les):
+def step_bytes_minimal(N, precision, les):
     """Algorithmic bytes per RK4 step, 4 [n_T (R + C) + 15 C] (minimal
     transform set, each transform one read + one write; RK state 15 C)."""
     R, C = field_bytes(N, precision)
@@ -352,7 +352,7 @@ def main(argv=None):
     if a.gpus != 1:
         print(
             f"bench.py runs on one GPU only: --gpus {a.gpus} is not supported "
-            "(this build has no multi-GPU path; see DESIGN.md for how it would shard)",
+            "(single-GPU implementation: no multi-GPU path)",
             file=sys.stderr,
         )
         return 2
@@ -375,7 +375,7 @@ def main(argv=None):
     ms_step = r64["ms_total"] / a.steps
     value = 1e3 / ms_step
     roof, per_kernel = roofline_of(r64, N, "fp64")
-    step_bytes = survey_step_bytes(N, "fp64", True)
+    step_bytes = step_bytes_minimal(N, "fp64", True)
     log(f"[bench] fp64 {value:.2f} steps/s ({ms_step:.3f} ms/step); dominant {roof['kernel']} "
         f"{roof['achieved']} GB/s")
     extra = {
