@@ -506,24 +506,35 @@ __global__ void __launch_bounds__((StridedShape<T, N, PL>::THREADS))
   };
   if constexpr (LES) {
     // H_i = F_y(G_i) + 2i Ky F_y(tau_iy): tau_xy (W4), tau_yy (W6), tau_yz (W7)
-#pragma unroll 1
-    for (int i = 0; i < 3; ++i) {
-      load(i == 0 ? 4 : (i == 1 ? 6 : 7));
-      FFT::run(v, lds, map, t, twr);
-      if (i == 0) store(4, v);
+    // fields in the order 4, 0, 6, 1, 7, 2, 3, 5; the next field's loads are
+    // issued before the current field's transform
+    auto field = [](int q) {
+      return q == 0 ? 4 : q == 1 ? 0 : q == 2 ? 6 : q == 3 ? 1 : q == 4 ? 7 : q == 5 ? 2 : q == 6 ? 3 : 5;
+    };
+    cpx<T> nx[P];
 #pragma unroll
-      for (int m = 0; m < P; ++m) acc[m] = mul_i((T(2) * (T)kfreq(t + m * NT, N)) * v[m]);
-      load(i);
-      FFT::run(v, lds, map, t, twr);
-#pragma unroll
-      for (int m = 0; m < P; ++m) acc[m] = v[m] + acc[m];
-      store(i, acc);
-    }
+    for (int m = 0; m < P; ++m) nx[m] = ldx(W + 4 * C, at(m));
 #pragma unroll 1
-    for (int r = 3; r <= 5; r += 2) {
-      load(r);
+    for (int q = 0; q < 8; ++q) {
+#pragma unroll
+      for (int m = 0; m < P; ++m) v[m] = nx[m];
+      if (q + 1 < 8) {
+        const int fn = field(q + 1);
+#pragma unroll
+        for (int m = 0; m < P; ++m) nx[m] = ldx(W + fn * C, at(m));
+      }
       FFT::run(v, lds, map, t, twr);
-      store(r, v);
+      if (q >= 6) {
+        store(field(q), v);
+      } else if ((q & 1) == 0) {
+        if (q == 0) store(4, v);
+#pragma unroll
+        for (int m = 0; m < P; ++m) acc[m] = mul_i((T(2) * (T)kfreq(t + m * NT, N)) * v[m]);
+      } else {
+#pragma unroll
+        for (int m = 0; m < P; ++m) acc[m] = v[m] + acc[m];
+        store(q >> 1, acc);
+      }
     }
   } else {
 #pragma unroll 1
