@@ -124,10 +124,10 @@ __global__ void __launch_bounds__(ZShape<N>::THREADS)
 // field's loads issued before the current field's transform.
 // ---------------------------------------------------------------------------
 template <typename T, int N, int DIR>
-__global__ void __launch_bounds__((StridedShape<T, N>::THREADS))
+__global__ void __launch_bounds__((WideShape<T, N>::THREADS))
     k_strided(const cpx<T>* in, cpx<T>* out, long outer, long M, long fstride, int nfields, T scale,
               const cpx<T>* __restrict__ tw, int xcd) {
-  using S = StridedShape<T, N>;
+  using S = WideShape<T, N>;
   constexpr int P = S::P, NT = S::NT, TK = S::TK;
   __shared__ cpx<T> lds[N * TK];
   const int c = threadIdx.x % TK, t = threadIdx.x / TK;
@@ -183,7 +183,7 @@ static int launch_z_c2r(const sles_plan_s* p, const void* in, void* out, long nr
 template <typename T, int N>
 static int launch_strided(const sles_plan_s* p, const void* in, void* out, long outer, long M,
                           long fstride, int nfields, int dir, double scale, hipStream_t s) {
-  using S = StridedShape<T, N>;
+  using S = WideShape<T, N>;
   const long blocks = (outer * M + S::TK - 1) / S::TK;
   if (dir < 0)
     hipLaunchKernelGGL((k_strided<T, N, -1>), dim3((unsigned)blocks), dim3(S::THREADS), 0, s, (const cpx<T>*)in,

@@ -90,6 +90,13 @@ struct StridedShape {
   static constexpr int THREADS = TK * NT;
 };
 
+// Strided passes of FP32 lines with >= 64 threads per line (radix-8 plans at
+// N >= 512): 1024-thread blocks, so that a tile still spans 16 columns
+// (128 B runs per row) -- their register use stays far below the 128-VGPR
+// limit of such blocks.  Other cases keep the default cap.
+template <typename T, int N, class PL = LinePlan<N>>
+using WideShape = StridedShape<T, N, PL, (sizeof(T) == 4 && PL::NT >= 64) ? 1024 : 0>;
+
 int z_c2r_rows(const sles_plan_s* p, const void* in, void* out, long nrows, double scale, hipStream_t s);
 
 }  // namespace sles

@@ -79,9 +79,9 @@ struct Cols {
 // 1. x-inverse
 // ---------------------------------------------------------------------------
 template <typename T, int N, int LES, class PL>
-__global__ void __launch_bounds__((StridedShape<T, N, PL>::THREADS))
+__global__ void __launch_bounds__((WideShape<T, N, PL>::THREADS))
     k_xinv(const cpx<T>* __restrict__ uh, cpx<T>* __restrict__ W, long C, const cpx<T>* __restrict__ tw, int xcd) {
-  using S = StridedShape<T, N, PL>;
+  using S = WideShape<T, N, PL>;
   constexpr int P = S::P, NT = S::NT, TK = S::TK;
   using FFT = LineFFTP<T, PL, +1>;
   __shared__ cpx<T> lds[N * TK];
@@ -149,9 +149,9 @@ __global__ void __launch_bounds__((StridedShape<T, N, PL>::THREADS))
 // 2. y-inverse (in place on W)
 // ---------------------------------------------------------------------------
 template <typename T, int N, int LES, class PL>
-__global__ void __launch_bounds__((StridedShape<T, N, PL>::THREADS))
+__global__ void __launch_bounds__((WideShape<T, N, PL>::THREADS))
     k_yinv(cpx<T>* W, long C, const cpx<T>* __restrict__ tw, int xcd) {
-  using S = StridedShape<T, N, PL>;
+  using S = WideShape<T, N, PL>;
   constexpr int P = S::P, NT = S::NT, TK = S::TK, NH = N / 2 + 1;
   using FFT = LineFFTP<T, PL, +1>;
   __shared__ cpx<T> lds[N * TK];
@@ -482,9 +482,9 @@ __global__ void __launch_bounds__(ZLines<PL>::THREADS)
 // 4. y-forward (in place)
 // ---------------------------------------------------------------------------
 template <typename T, int N, int LES, class PL>
-__global__ void __launch_bounds__((StridedShape<T, N, PL>::THREADS))
+__global__ void __launch_bounds__((WideShape<T, N, PL>::THREADS))
     k_yfwd(cpx<T>* W, long C, const cpx<T>* __restrict__ tw, int xcd) {
-  using S = StridedShape<T, N, PL>;
+  using S = WideShape<T, N, PL>;
   constexpr int P = S::P, NT = S::NT, TK = S::TK, NH = N / 2 + 1;
   using FFT = LineFFTP<T, PL, -1>;
   __shared__ cpx<T> lds[N * TK];
@@ -796,7 +796,7 @@ struct ProfScope {
 // ---------------------------------------------------------------------------
 template <typename T, int N, class PS, class PX, class PZ>
 struct Fused {
-  using SS = StridedShape<T, N, PS>;  // x/y inverse and y forward
+  using SS = WideShape<T, N, PS>;  // x/y inverse and y forward
   using SX = StridedShape<T, N, PX, 512>;  // x forward + stage update
   static constexpr int ZB = (int)(((long)N * N + ZLines<PZ>::LPB - 1) / ZLines<PZ>::LPB);
 
