@@ -211,7 +211,8 @@ size_t sles_step_work_bytes(sles_plan plan, int les_model);
  * synchronises on them, adds the elapsed milliseconds and launch counts per
  * kernel kind (0 x-inverse, 1 y-inverse, 2 z, 3 y-forward, 4 x-forward with
  * the stage update, 5 CFL reduction, 6 U/curl output) into ms[]/count[]
- * (nkinds entries) and clears the record. */
+ * (nkinds entries) and clears the record.  The profiling state is global to
+ * the process and not thread-safe; do not enable it under HIP graph capture. */
 int sles_profile_enable(int on);
 int sles_profile_read(double* ms, long* count, int nkinds);
 int sles_step(sles_plan plan, int scheme, const sles_step_params* params, void* u_hat, void* u_hat0, void* u_hat1,

@@ -99,6 +99,36 @@ def precision_study(N=256, checkpoints=(100,), seed=1, les="Smagorinsky", Cs=0.1
     return out
 
 
+def precision_tgv(N=64, steps=200, nu=1 / 1600, dt=0.01):
+    """FP32 form against the FP64 form on the Taylor-Green case (no LES, RK4,
+    fixed dt): velocity rel-L2 and max relative E(k) error after `steps`."""
+    import numpy as np
+
+    from .initial import initial_velocity
+
+    x = np.arange(N) * 2 * np.pi / N
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    u, v, w = np.sin(X) * np.cos(Y) * np.cos(Z), -np.cos(X) * np.sin(Y) * np.cos(Z), np.zeros_like(X)
+    doms = []
+    for prec in ("fp64", "fp32"):
+        d = InitializeDomain(N=N, nu=nu, time_stepSize=dt, method="RK4", precision=prec)
+        initial_velocity(d, u, v, w)
+        d.U_hat1.copy_(d.U_hat)
+        doms.append(d)
+    for _ in range(steps):
+        for d in doms:
+            fused.rk4_step(d, dt=dt)
+    d64, d32 = doms
+    e64 = shell_energy(d64.U_hat, N)[1:]
+    e32 = shell_energy(d32.U_hat, N)[1:]
+    resolved = e64 >= 1e-12 * e64.max()
+    rel = ((e32 - e64).abs() / e64)[resolved]
+    res = {"case": "taylor_green", "N": N, "steps": steps, "velocity_rel_l2": _rel_l2(d32._U, d64._U),
+           "max_rel_Ek_error_resolved_shells": float(rel.max()), "resolved_kmax": int(resolved.nonzero().max()) + 1}
+    print(json.dumps(res), flush=True)
+    return res
+
+
 def spectral_cut(U_hat, N_from, N_to):
     """Keep |kx|, |ky| < N_to/2 and kz <= N_to/2 of an N_from^3 spectrum, scaled
     by (N_to/N_from)^3 for the unscaled forward-transform convention."""
@@ -308,6 +338,9 @@ def main(argv=None):
     r = sub.add_parser("ensemble")
     r.add_argument("--steps", type=int, default=200)
     r.add_argument("--out", default=None)
+    pt = sub.add_parser("precision-tgv")
+    pt.add_argument("--n", type=int, default=64)
+    pt.add_argument("--steps", type=int, default=200)
     rc = sub.add_parser("adjoint-cost")
     rc.add_argument("--n", type=int, default=256)
     rc.add_argument("--scheme", default="RK4")
@@ -323,6 +356,9 @@ def main(argv=None):
         if a.out:
             with open(a.out, "w") as f:
                 json.dump(res, f, indent=1)
+        return
+    if a.cmd == "precision-tgv":
+        precision_tgv(a.n, a.steps)
         return
     if a.cmd == "adjoint-cost":
         adjoint_cost(a.n, a.scheme, a.les, a.precision, a.steps)
