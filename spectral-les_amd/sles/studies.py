@@ -337,6 +337,7 @@ def main(argv=None):
     q.add_argument("--out", default=None)
     r = sub.add_parser("ensemble")
     r.add_argument("--steps", type=int, default=200)
+    r.add_argument("--steps-per-graph", type=int, default=10)
     r.add_argument("--out", default=None)
     pt = sub.add_parser("precision-tgv")
     pt.add_argument("--n", type=int, default=64)
@@ -373,11 +374,13 @@ def main(argv=None):
         from .ensemble import Ensemble
 
         ens = Ensemble(((64, 8), (96, 8), (128, 8)), precision="fp32", seed=0)
-        ens.capture(steps_per_graph=1)
+        ens.capture(steps_per_graph=a.steps_per_graph)
         rate = ens.run(a.steps)
         res = {"members": [[d.N, d.Cs] for d in ens.members], "steps": a.steps,
-               "grid_points_x_steps_per_s": rate, "dt_last": ens.dt_values()}
-        print(json.dumps({"grid_points_x_steps_per_s": rate, "steps": a.steps}), flush=True)
+               "steps_per_graph": a.steps_per_graph, "grid_points_x_steps_per_s": rate,
+               "dt_last": ens.dt_values()}
+        print(json.dumps({"grid_points_x_steps_per_s": rate, "steps": a.steps,
+                          "steps_per_graph": a.steps_per_graph}), flush=True)
         if a.out:
             with open(a.out, "w") as f:
                 json.dump(res, f, indent=1)
