@@ -344,7 +344,9 @@ __device__ __forceinline__ void line_stage(cpx<T> (&v)[P], int t, const TW& tw) 
   }
 }
 
-template <typename T, int N, int P, int R, int Ns, bool WAVE, class Map>
+// TRAIL: end with a second barrier, so the buffer can be written again at
+// once.  Without it the next write must go to another buffer (ping-pong).
+template <typename T, int N, int P, int R, int Ns, bool WAVE, bool TRAIL, class Map>
 __device__ __forceinline__ void line_exchange(cpx<T> (&v)[P], cpx<T>* lds, const Map& map, int t) {
   constexpr int NT = N / P;
   constexpr int Q = P / R;
@@ -359,42 +361,50 @@ __device__ __forceinline__ void line_exchange(cpx<T> (&v)[P], cpx<T>* lds, const
   line_sync<WAVE>();
 #pragma unroll
   for (int m = 0; m < P; ++m) v[m] = lds[map(t + m * NT)];
-  line_sync<WAVE>();
+  if constexpr (TRAIL) line_sync<WAVE>();
 }
 
-template <typename T, int N, int P, int DIR, bool WAVE, int Ns, int OFF, int... Rs>
+// PPB > 0: exchanges alternate between lds and lds + PPB (two buffers of PPB
+// elements), so only the last exchange needs its trailing barrier: a thread
+// writes a buffer again only after the barrier of the exchange in between,
+// which every thread reaches after its reads of that buffer.  EX: index of
+// the next exchange.
+template <typename T, int N, int P, int DIR, bool WAVE, int PPB, int Ns, int OFF, int EX, int... Rs>
 struct LineStages;
 
-template <typename T, int N, int P, int DIR, bool WAVE, int Ns, int OFF, int R>
-struct LineStages<T, N, P, DIR, WAVE, Ns, OFF, R> {
+template <typename T, int N, int P, int DIR, bool WAVE, int PPB, int Ns, int OFF, int EX, int R>
+struct LineStages<T, N, P, DIR, WAVE, PPB, Ns, OFF, EX, R> {
   template <class Map, class TW>
   __device__ __forceinline__ static void run(cpx<T> (&v)[P], cpx<T>* lds, const Map& map, int t, const TW& tw) {
     line_stage<T, N, P, DIR, R, Ns, OFF>(v, t, tw);
   }
 };
 
-template <typename T, int N, int P, int DIR, bool WAVE, int Ns, int OFF, int R, int R2, int... Rest>
-struct LineStages<T, N, P, DIR, WAVE, Ns, OFF, R, R2, Rest...> {
+template <typename T, int N, int P, int DIR, bool WAVE, int PPB, int Ns, int OFF, int EX, int R, int R2, int... Rest>
+struct LineStages<T, N, P, DIR, WAVE, PPB, Ns, OFF, EX, R, R2, Rest...> {
   static constexpr int NEXT = OFF + (Ns > 1 ? (P / R) * (R - 1) : 0);
+  static constexpr bool LAST = sizeof...(Rest) == 0;
   template <class Map, class TW>
   __device__ __forceinline__ static void run(cpx<T> (&v)[P], cpx<T>* lds, const Map& map, int t, const TW& tw) {
     line_stage<T, N, P, DIR, R, Ns, OFF>(v, t, tw);
-    line_exchange<T, N, P, R, Ns, WAVE>(v, lds, map, t);
-    LineStages<T, N, P, DIR, WAVE, Ns * R, NEXT, R2, Rest...>::run(v, lds, map, t, tw);
+    cpx<T>* const buf = (PPB > 0 && (EX & 1)) ? lds + PPB : lds;
+    line_exchange<T, N, P, R, Ns, WAVE, PPB == 0 || LAST>(v, buf, map, t);
+    LineStages<T, N, P, DIR, WAVE, PPB, Ns * R, NEXT, EX + 1, R2, Rest...>::run(v, lds, map, t, tw);
   }
 };
 
 // WAVE: every line of the block lies inside one wavefront (NT <= 64 and
 // 64 % NT == 0, lines packed by thread index), so the exchanges need only
 // wavefront synchronisation.
-template <typename T, class PL, int DIR, bool WAVE = false>
+// PPB > 0: lds holds two buffers of PPB elements each (see LineStages).
+template <typename T, class PL, int DIR, bool WAVE = false, int PPB = 0>
 struct LineFFTP {
   static constexpr int N = PL::N;
   static constexpr int P = PL::P;
   static constexpr int NT = PL::NT;
   static_assert(!WAVE || (NT <= 64 && 64 % NT == 0), "wavefront-local lines need NT | 64");
   template <int... Rs>
-  using stages = LineStages<T, N, P, DIR, WAVE, 1, 0, Rs...>;
+  using stages = LineStages<T, N, P, DIR, WAVE, PPB, 1, 0, 0, Rs...>;
   using S = typename PL::template apply<stages>;
   // Transform the line held by thread t (0 <= t < NT).  Every thread of the
   // block (WAVE: of the wavefront) must call this: it contains barriers when

@@ -83,8 +83,9 @@ __global__ void __launch_bounds__((WideShape<T, N, PL>::THREADS))
     k_xinv(const cpx<T>* __restrict__ uh, cpx<T>* __restrict__ W, long C, const cpx<T>* __restrict__ tw, int xcd) {
   using S = WideShape<T, N, PL>;
   constexpr int P = S::P, NT = S::NT, TK = S::TK;
-  using FFT = LineFFTP<T, PL, +1>;
-  __shared__ cpx<T> lds[N * TK];
+  constexpr int PPB = N * TK;  // ping-pong exchange buffers
+  using FFT = LineFFTP<T, PL, +1, false, PPB>;
+  __shared__ cpx<T> lds[N * TK + PPB];
   const Cols<N, TK> col(xcd);
   const int t = threadIdx.x / TK;
   const TwRegs<T, PL> twr(tw, t);  // read once per block, used by every transform
@@ -153,8 +154,9 @@ __global__ void __launch_bounds__((WideShape<T, N, PL>::THREADS))
     k_yinv(cpx<T>* W, long C, const cpx<T>* __restrict__ tw, int xcd) {
   using S = WideShape<T, N, PL>;
   constexpr int P = S::P, NT = S::NT, TK = S::TK, NH = N / 2 + 1;
-  using FFT = LineFFTP<T, PL, +1>;
-  __shared__ cpx<T> lds[N * TK];
+  constexpr int PPB = N * TK;  // ping-pong exchange buffers
+  using FFT = LineFFTP<T, PL, +1, false, PPB>;
+  __shared__ cpx<T> lds[N * TK + PPB];
   const Cols<N, TK> col(xcd);
   const int t = threadIdx.x / TK;
   const TwRegs<T, PL> twr(tw, t);  // read once per block, used by every transform
@@ -486,8 +488,11 @@ __global__ void __launch_bounds__((WideShape<T, N, PL>::THREADS))
     k_yfwd(cpx<T>* W, long C, const cpx<T>* __restrict__ tw, int xcd) {
   using S = WideShape<T, N, PL>;
   constexpr int P = S::P, NT = S::NT, TK = S::TK, NH = N / 2 + 1;
-  using FFT = LineFFTP<T, PL, -1>;
-  __shared__ cpx<T> lds[N * TK];
+  // ping-pong exchange buffers in FP32 only: in FP64 they would double the
+  // block's LDS to 128 KiB, which measured slower
+  constexpr int PPB = sizeof(T) == 4 ? N * TK : 0;
+  using FFT = LineFFTP<T, PL, -1, false, PPB>;
+  __shared__ cpx<T> lds[N * TK + PPB];
   const Cols<N, TK> col(xcd);
   const int t = threadIdx.x / TK;
   const TwRegs<T, PL> twr(tw, t);  // read once per block, used by every transform
@@ -665,8 +670,9 @@ __global__ void __launch_bounds__((StridedShape<T, N, PL, 512>::THREADS))
            const cpx<T>* __restrict__ tw, int xcd) {
   using S = StridedShape<T, N, PL, 512>;
   constexpr int P = S::P, NT = S::NT, TK = S::TK, NH = N / 2 + 1;
-  using FFT = LineFFTP<T, PL, -1>;
-  __shared__ cpx<T> lds[N * TK];
+  constexpr int PPB = N * TK;  // ping-pong exchange buffers
+  using FFT = LineFFTP<T, PL, -1, false, PPB>;
+  __shared__ cpx<T> lds[N * TK + PPB];
   double dt = a.dt;
   if (a.dt_dev) {
     dt = *a.dt_dev;
