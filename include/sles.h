@@ -221,8 +221,9 @@ int sles_step(sles_plan plan, int scheme, const sles_step_params* params, void* 
 /* One fused RK4 step (as sles_step with SLES_SCHEME_RK4, host dt, no physical
  * output) that also writes the stage states Y_1, Y_2, Y_3 (the arguments of
  * the second to fourth RHS evaluations, 3 spectral fields each) to stages
- * (9 fields).  u_hat1 must equal u_hat on entry, as for sles_step.  No
- * counterpart in the reference: the adjoint's recomputation of a step. */
+ * (9 fields).  u_hat0 receives a copy of u_hat (Y_0); u_hat1 is written at
+ * the first stage before it is read, so its content on entry is not used.
+ * No counterpart in the reference: the adjoint's recomputation of a step. */
 int sles_rk4_stage_states(sles_plan plan, const sles_step_params* params, void* u_hat, void* u_hat0, void* u_hat1,
                           void* stages, void* work, void* stream);
 
@@ -239,6 +240,14 @@ int sles_rk4_stage_states(sles_plan plan, const sles_step_params* params, void* 
 size_t sles_adjoint_work_bytes(sles_plan plan, int les_model);
 int sles_adjoint_rhs(sles_plan plan, const sles_step_params* params, const void* y, const void* mu, void* out,
                      int accumulate, int linear, double dcoeff, double* cs_grad, void* work, void* stream);
+/* The RK4 adjoint stage in one call: sles_adjoint_rhs with linear = 1 and
+ * accumulate = 0 for
+ *   mu = a x + b z   (z may be NULL, taken as 0)
+ * formed inside the adjoint passes (no separate combination pass).  out must
+ * not alias y, x or z; `work` as for sles_adjoint_rhs. */
+int sles_adjoint_rhs_combined(sles_plan plan, const sles_step_params* params, const void* y, double a, const void* x,
+                              double b, const void* z, void* out, double dcoeff, double* cs_grad, void* work,
+                              void* stream);
 /* ETD2 adjoint combination (Cox-Matthews, L = -nu K^2, h = dt, FP64 phi):
  *   mu = h (phi1 + phi2)(hL) lam1 - h phi2(hL) lam2 ;  lam_out = e^{hL} lam1
  * lam2 may be NULL (taken as 0).  3 spectral fields each. */
@@ -253,6 +262,10 @@ int sles_etd2_adjoint_combine(sles_plan plan, double dt, double nu, const void* 
  * inputs; `work` as for sles_adjoint_rhs. */
 int sles_etd2_adjoint_rhs(sles_plan plan, const sles_step_params* params, const void* y, const void* lam1,
                           const void* lam2, void* out, double dcoeff, double* cs_grad, void* work, void* stream);
+/* out = terms[0] + terms[1] + ... (in that order) over nfields spectral
+ * fields, 1 <= nterms <= 8 (terms: host array of device pointers; out may be
+ * one of the terms). */
+int sles_sum_fields(sles_plan plan, int nfields, int nterms, const void* const* terms, void* out, void* stream);
 /* y = a x + b y over nfields spectral fields (contiguous). */
 int sles_axpby(sles_plan plan, int nfields, double a, const void* x, double b, void* y, void* stream);
 /* out = a x + b y over nfields spectral fields (out may be x or y). */

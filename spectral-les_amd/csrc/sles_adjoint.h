@@ -13,9 +13,9 @@
 //   projection Pi = I - K K^T/K^2 (per k):        Pi (symmetric)
 // with Wz = 2 for 0 < kz < N/2 and 1 on the kz = 0, N/2 planes.
 //
-// Passes (workspace A of 17 spectral fields):
+// Passes (workspace A of 17 spectral fields, plus A17-19 for a combined mu):
 //   1 x-inverse   Y -> A0-2 = Y, A3-5 = i Kx Y ;  g = dealias Pi mu -> A9-11 = g,
-//                 A12-14 = -2i Kx g
+//                 A12-14 = -2i Kx g ; mu = c1 x + c2 z formed here -> A17-19
 //   2 y-inverse   A0-8 as the forward LES pass; A9-11 = g, A12 = tau_bar_xx,
 //                 A13 = tau_bar_xy, A14 = tau_bar_xz (Kx part), A15 = tau_bar_yy,
 //                 A16 = tau_bar_yz (Ky part)
@@ -34,30 +34,38 @@
 #include "sles_rhs.h"
 
 namespace sles {
+constexpr int kAdjFields = 20;  // spectral fields of the adjoint workspace
+}
+
+namespace sles {
 
 template <typename T>
 __device__ __forceinline__ cpx<T> mul_mis(T s, cpx<T> a) {  // (-i s) a
   return {s * a.y, -s * a.x};
 }
 
-// Where the x-inverse and x-forward passes take the adjoint mu from: the
-// array mu, or (ETD2 backward step) mu = w1 lam1 - w0 lam2 formed per mode,
-// with the output out = E lam1 + J^T mu (E = e^{hL}, L = -nu K^2,
-// w1 = h (phi1 + phi2), w0 = h phi2).
+// Where the x-inverse and x-forward passes take the adjoint mu from:
+//   etd2 = 0: mu = c1 lam1 + c2 lam2 (lam2 may be null; c1 = 1, lam2 = null
+//             is mu = lam1 as given);
+//   etd2 = 1: mu = w1 lam1 - w0 lam2 formed per mode (ETD2 backward step),
+//             with the output out = E lam1 + J^T mu (E = e^{hL}, L = -nu K^2,
+//             w1 = h (phi1 + phi2), w0 = h phi2).
 struct AdjSrc {
-  const void* mu;
   const void* lam1;
   const void* lam2;  // may be null (treated as zero)
+  double c1, c2;
   double h;
   int etd2;
 };
+// mw (etd2 = 0 only, may be null): where the x-inverse pass stores the mu it
+// forms, for the x-forward pass's linear term (which then reads one array).
 template <typename T>
 struct MuSrc {
-  const cpx<T>* mu;
   const cpx<T>* l1;
   const cpx<T>* l2;
+  T c1, c2;
   double h, nu;
-  int etd2;
+  cpx<T>* mw;
   __device__ __forceinline__ void weights(long k2, T& w1, T& w0, T& E) const {
     const double z = -h * nu * (double)k2;
     const double p1 = phi1_d(z), p2 = phi2_d(z);
@@ -69,7 +77,9 @@ struct MuSrc {
   template <bool ETD2>
   __device__ __forceinline__ cpx<T> at(int j, long C, long e, T w1, T w0) const {
     if constexpr (!ETD2) {
-      return mu[j * C + e];
+      cpx<T> r = c1 * l1[j * C + e];
+      if (l2) r = r + c2 * l2[j * C + e];
+      return r;
     } else {
       cpx<T> r = w1 * l1[j * C + e];
       if (l2) r = r - w0 * l2[j * C + e];
@@ -101,8 +111,9 @@ __global__ void __launch_bounds__((StridedShape<T, N, PL>::THREADS))
                const cpx<T>* __restrict__ tw, int xcd) {
   using S = StridedShape<T, N, PL>;
   constexpr int P = S::P, NT = S::NT, TK = S::TK, NH = N / 2 + 1;
-  using FFT = LineFFTP<T, PL, +1>;
-  __shared__ cpx<T> lds[N * TK];
+  constexpr int PPB = N * TK;  // ping-pong exchange buffers
+  using FFT = LineFFTP<T, PL, +1, false, PPB>;
+  __shared__ cpx<T> lds[N * TK + PPB];
   const Cols<N, TK> col(xcd);
   const int t = threadIdx.x / TK;
   // FP64 radix-8 ETD2 variant: table twiddles (the ETD2 weights need the registers)
@@ -144,6 +155,13 @@ __global__ void __launch_bounds__((StridedShape<T, N, PL>::THREADS))
     if constexpr (ETD2) mu.weights(k2, a1, a0, E);
     const cpx<T> m0 = mu.template at<ETD2>(0, C, e, a1, a0), m1 = mu.template at<ETD2>(1, C, e, a1, a0),
                  m2 = mu.template at<ETD2>(2, C, e, a1, a0);
+    if constexpr (!ETD2) {
+      if (mu.mw && col.valid) {
+        mu.mw[e] = m0;
+        mu.mw[C + e] = m1;
+        mu.mw[2 * C + e] = m2;
+      }
+    }
     if constexpr (HOLD) {
       mh0[m] = m0;
       mh1[m] = m1;
@@ -196,8 +214,9 @@ __global__ void __launch_bounds__((StridedShape<T, N, PL>::THREADS))
     k_adj_yinv(cpx<T>* A, long C, const cpx<T>* __restrict__ tw, int xcd) {
   using S = StridedShape<T, N, PL>;
   constexpr int P = S::P, NT = S::NT, TK = S::TK, NH = N / 2 + 1;
-  using FFT = LineFFTP<T, PL, +1>;
-  __shared__ cpx<T> lds[N * TK];
+  constexpr int PPB = N * TK;  // ping-pong exchange buffers
+  using FFT = LineFFTP<T, PL, +1, false, PPB>;
+  __shared__ cpx<T> lds[N * TK + PPB];
   const Cols<N, TK> col(xcd);
   const int t = threadIdx.x / TK;
   const TwRegs<T, PL> twr(tw, t);  // read once per block, used by every transform
@@ -543,8 +562,9 @@ __global__ void __launch_bounds__((StridedShape<T, N, PL>::THREADS))
     k_adj_yfwd(cpx<T>* A, long C, const cpx<T>* __restrict__ tw, int xcd) {
   using S = StridedShape<T, N, PL>;
   constexpr int P = S::P, NT = S::NT, TK = S::TK, NH = N / 2 + 1;
-  using FFT = LineFFTP<T, PL, -1>;
-  __shared__ cpx<T> lds[N * TK];
+  constexpr int PPB = N * TK;  // ping-pong exchange buffers
+  using FFT = LineFFTP<T, PL, -1, false, PPB>;
+  __shared__ cpx<T> lds[N * TK + PPB];
   const Cols<N, TK> col(xcd);
   const int t = threadIdx.x / TK;
   const TwRegs<T, PL> twr(tw, t);  // read once per block, used by every transform
@@ -592,8 +612,9 @@ __global__ void __launch_bounds__((StridedShape<T, N, PL>::THREADS))
                const cpx<T>* __restrict__ tw, int xcd) {
   using S = StridedShape<T, N, PL>;
   constexpr int P = S::P, NT = S::NT, TK = S::TK, NH = N / 2 + 1;
-  using FFT = LineFFTP<T, PL, -1>;
-  __shared__ cpx<T> lds[N * TK];
+  constexpr int PPB = N * TK;  // ping-pong exchange buffers
+  using FFT = LineFFTP<T, PL, -1, false, PPB>;
+  __shared__ cpx<T> lds[N * TK + PPB];
   const Cols<N, TK> col(xcd);
   const int t = threadIdx.x / TK;
   // FP64 radix-8 ETD2 variant: table twiddles (the ETD2 weights need the registers)
@@ -631,7 +652,7 @@ __global__ void __launch_bounds__((StridedShape<T, N, PL>::THREADS))
           if (a.linear) {
             const double visc = a.linear == 1 ? a.nu * (double)k2 : 0.0;
             const T phi = (T)((a.forcing > 0.0 && k2 != 0 ? a.forcing : 0.0) - visc);
-            res = res + phi * mu.mu[e];
+            res = res + phi * mu.l1[e];  // mu as given, or as stored by the x-inverse pass
           }
           out[e] = a.accumulate ? out[e] + res : res;
         }
@@ -649,12 +670,14 @@ int adjoint_impl(const sles_plan_s* p, const sles_step_params* prm, const void* 
   const cpx<T>* tw = (const cpx<T>*)p->tw;
   const int xcd = p->tune_xcd;
   cpx<T>* A = (cpx<T>*)work;
-  double* part = (double*)(A + 17 * C);
+  double* part = (double*)(A + kAdjFields * C);
   constexpr long NCOL = (long)N * (N / 2 + 1);  // columns of every strided pass
   const dim3 gs((unsigned)((NCOL + SS::TK - 1) / SS::TK)), bs(SS::THREADS);
   const T scale = T(1.0 / ((double)N * N * N));
-  const MuSrc<T> ms{(const cpx<T>*)src.mu, (const cpx<T>*)src.lam1, (const cpx<T>*)src.lam2, src.h, prm->nu,
-                    src.etd2};
+  // a combined mu that the x-forward pass needs is stored once, in A[17..19]
+  cpx<T>* const mw = (!src.etd2 && linear && (src.lam2 || src.c1 != 1.0)) ? A + 17 * C : nullptr;
+  const MuSrc<T> ms{(const cpx<T>*)src.lam1, (const cpx<T>*)src.lam2, (T)src.c1, (T)src.c2, src.h, prm->nu, mw};
+  const MuSrc<T> msx = mw ? MuSrc<T>{mw, nullptr, T(1), T(0), src.h, prm->nu, nullptr} : ms;
   if (src.etd2)
     hipLaunchKernelGGL((k_adj_xinv<T, N, PS, true>), gs, bs, 0, s, (const cpx<T>*)y, ms, A, C, tw, xcd);
   else
@@ -681,7 +704,7 @@ int adjoint_impl(const sles_plan_s* p, const sles_step_params* prm, const void* 
   if (src.etd2)
     hipLaunchKernelGGL((k_adj_xfwd<T, N, PS, true>), gs, bs, 0, s, (const cpx<T>*)A, C, ms, (cpx<T>*)out, aa, tw, xcd);
   else
-    hipLaunchKernelGGL((k_adj_xfwd<T, N, PS, false>), gs, bs, 0, s, (const cpx<T>*)A, C, ms, (cpx<T>*)out, aa, tw,
+    hipLaunchKernelGGL((k_adj_xfwd<T, N, PS, false>), gs, bs, 0, s, (const cpx<T>*)A, C, msx, (cpx<T>*)out, aa, tw,
                        xcd);
   return check_launch("sles_adjoint_rhs");
 }

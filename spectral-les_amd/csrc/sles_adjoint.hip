@@ -11,6 +11,20 @@ __global__ void k_axpby(long n, double a, const cpx<T>* x, double b, const cpx<T
     out[e] = (T)a * x[e] + (T)b * y[e];
 }
 
+// out = t_0 + t_1 + ... + t_{n-1}, summed in that order (out may be a term)
+struct SumTerms {
+  const void* t[8];
+  int n;
+};
+template <typename T>
+__global__ void k_sum_fields(long n, SumTerms st, cpx<T>* out) {
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+    cpx<T> r = ((const cpx<T>*)st.t[0])[e];
+    for (int i = 1; i < st.n; ++i) r = r + ((const cpx<T>*)st.t[i])[e];
+    out[e] = r;
+  }
+}
+
 template <typename T>
 __global__ void k_etd2_adj(long nc, int n, int nh, double h, double nu, const cpx<T>* l1, const cpx<T>* l2,
                            cpx<T>* mu, cpx<T>* lo) {
@@ -37,7 +51,7 @@ size_t sles_adjoint_work_bytes(sles_plan p, int les_model) {
   (void)les_model;
   if (!p) return 0;
   const size_t esz = p->precision == SLES_FP64 ? 16 : 8;
-  return 17 * (size_t)ncplx(p) * esz + sizeof(double) * ((size_t)p->n * p->n + 64);
+  return kAdjFields * (size_t)ncplx(p) * esz + sizeof(double) * ((size_t)p->n * p->n + 64);
 }
 
 int sles_adjoint_rhs(sles_plan p, const sles_step_params* prm, const void* y, const void* mu, void* out,
@@ -47,9 +61,22 @@ int sles_adjoint_rhs(sles_plan p, const sles_step_params* prm, const void* y, co
   if (out == mu || out == y) return fail("sles_adjoint_rhs: out must not alias y or mu");
   if (!sles_supported_n(p->n)) return fail("sles_adjoint_rhs: unsupported grid size");
   hipStream_t s = (hipStream_t)stream;
-  const AdjSrc src{mu, nullptr, nullptr, 0.0, 0};
+  const AdjSrc src{mu, nullptr, 1.0, 0.0, 0.0, 0};
   return p->precision == SLES_FP64 ? adjoint_f64(p, prm, y, src, out, accumulate, linear, dcoeff, cs_grad, work, s)
                                    : adjoint_f32(p, prm, y, src, out, accumulate, linear, dcoeff, cs_grad, work, s);
+}
+
+int sles_adjoint_rhs_combined(sles_plan p, const sles_step_params* prm, const void* y, double a, const void* x,
+                              double b, const void* z, void* out, double dcoeff, double* cs_grad, void* work,
+                              void* stream) {
+  if (!p || !prm || !y || !x || !out || !work) return fail("sles_adjoint_rhs_combined: null argument");
+  if (prm->les_model < 0 || prm->les_model > 2) return fail("sles_adjoint_rhs_combined: les_model must be 0, 1 or 2");
+  if (out == x || out == z || out == y) return fail("sles_adjoint_rhs_combined: out must not alias y, x or z");
+  if (!sles_supported_n(p->n)) return fail("sles_adjoint_rhs_combined: unsupported grid size");
+  hipStream_t s = (hipStream_t)stream;
+  const AdjSrc src{x, z, a, b, 0.0, 0};
+  return p->precision == SLES_FP64 ? adjoint_f64(p, prm, y, src, out, 0, 1, dcoeff, cs_grad, work, s)
+                                   : adjoint_f32(p, prm, y, src, out, 0, 1, dcoeff, cs_grad, work, s);
 }
 
 int sles_etd2_adjoint_rhs(sles_plan p, const sles_step_params* prm, const void* y, const void* lam1,
@@ -60,7 +87,7 @@ int sles_etd2_adjoint_rhs(sles_plan p, const sles_step_params* prm, const void* 
   if (!(prm->dt > 0.0)) return fail("sles_etd2_adjoint_rhs: dt must be positive");
   if (!sles_supported_n(p->n)) return fail("sles_etd2_adjoint_rhs: unsupported grid size");
   hipStream_t s = (hipStream_t)stream;
-  const AdjSrc src{nullptr, lam1, lam2, prm->dt, 1};
+  const AdjSrc src{lam1, lam2, 0.0, 0.0, prm->dt, 1};
   return p->precision == SLES_FP64 ? adjoint_f64(p, prm, y, src, out, 0, 2, dcoeff, cs_grad, work, s)
                                    : adjoint_f32(p, prm, y, src, out, 0, 2, dcoeff, cs_grad, work, s);
 }
@@ -95,6 +122,27 @@ static int axpby_launch(sles_plan p, int nfields, double a, const void* x, doubl
     hipLaunchKernelGGL(k_axpby<float>, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, n, a,
                        (const cpx<float>*)x, b, (const cpx<float>*)y, (cpx<float>*)out);
   return check_launch(what);
+}
+
+int sles_sum_fields(sles_plan p, int nfields, int nterms, const void* const* terms, void* out, void* stream) {
+  if (!p || !terms || !out || nfields < 1) return fail("sles_sum_fields: bad argument");
+  if (nterms < 1 || nterms > 8) return fail("sles_sum_fields: nterms must be 1..8");
+  SumTerms st{};
+  st.n = nterms;
+  for (int i = 0; i < nterms; ++i) {
+    if (!terms[i]) return fail("sles_sum_fields: null term");
+    st.t[i] = terms[i];
+  }
+  const long n = (long)nfields * ncplx(p);
+  long blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  if (p->precision == SLES_FP64)
+    hipLaunchKernelGGL(k_sum_fields<double>, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, n, st,
+                       (cpx<double>*)out);
+  else
+    hipLaunchKernelGGL(k_sum_fields<float>, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, n, st,
+                       (cpx<float>*)out);
+  return check_launch("sles_sum_fields");
 }
 
 int sles_axpby(sles_plan p, int nfields, double a, const void* x, double b, void* y, void* stream) {

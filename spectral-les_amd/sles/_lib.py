@@ -62,6 +62,8 @@ _PROTOS = {
     "sles_lincomb": (_int, [_vp, _int, _dbl, _vp, _dbl, _vp, _vp, _vp]),
     "sles_etd2_adjoint_combine": (_int, [_vp, _dbl, _dbl, _vp, _vp, _vp, _vp, _vp]),
     "sles_etd2_adjoint_rhs": (_int, [_vp, _vp, _vp, _vp, _vp, _vp, _dbl, _vp, _vp, _vp]),
+    "sles_adjoint_rhs_combined": (_int, [_vp, _vp, _vp, _dbl, _vp, _dbl, _vp, _vp, _dbl, _vp, _vp, _vp]),
+    "sles_sum_fields": (_int, [_vp, _int, _int, _vp, _vp, _vp]),
     "sles_profile_enable": (_int, [_int]),
     "sles_profile_read": (_int, [_vp, _vp, _int]),
     "sles_step": (_int, [_vp, _int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),

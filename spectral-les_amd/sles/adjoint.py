@@ -39,7 +39,7 @@ RK4_B = (0.5, 0.5, 1.0)
 
 class DifferentiableLES:
     def __init__(self, N, nu, dt, nsteps, les_model="Smagorinsky", scheme="RK4", precision="fp32",
-                 checkpoint_every=10, forcing=0.0, device="cuda"):
+                 checkpoint_every=10, forcing=0.0, device="cuda", keep_last_segment=True):
         from .initialize_domain import InitializeDomain
 
         if scheme not in ("RK4", "ETD2"):
@@ -50,6 +50,9 @@ class DifferentiableLES:
         self.N, self.nu, self.dt, self.nsteps = N, nu, dt, int(nsteps)
         self.scheme = scheme
         self.K = max(1, int(checkpoint_every))
+        # the forward pass keeps what the backward sweep needs for the last
+        # segment (RK4: 12 fields per step, ETD2: 3), so it is not recomputed
+        self.keep_last_segment = bool(keep_last_segment)
         self.les_model = les_model
         self.adj_work = torch.empty(int(_lib.lib.sles_adjoint_work_bytes(self.d.plan.handle, self.d.sgs_model)),
                                     dtype=torch.uint8, device=self.d.device)
@@ -86,28 +89,46 @@ class DifferentiableLES:
                   out.data_ptr(), 1 if accumulate else 0, int(linear), self.dcoeff(),
                   self.cs_grad.data_ptr() if d.LES_ON else None, self.adj_work.data_ptr(), d.stream)
 
+    def vjp_combined(self, Y, a, x, b, z, out):
+        """out = J(Y)^T (a x + b z) (z may be None): one fused call
+        (sles_adjoint_rhs_combined); accumulates d/dCs like vjp."""
+        d = self.d
+        prm = fused.params(d, dt=self.dt)
+        _lib.call("sles_adjoint_rhs_combined", d.plan.handle, ctypes.byref(prm), Y.data_ptr(), float(a), x.data_ptr(),
+                  float(b), z.data_ptr() if z is not None else None, out.data_ptr(), self.dcoeff(),
+                  self.cs_grad.data_ptr() if d.LES_ON else None, self.adj_work.data_ptr(), d.stream)
+
+    def sum_fields(self, terms, out):
+        """out = terms[0] + terms[1] + ... (3 spectral fields each)."""
+        ptrs = (ctypes.c_void_p * len(terms))(*[t.data_ptr() for t in terms])
+        _lib.call("sles_sum_fields", self.d.plan.handle, 3, len(terms), ptrs, out.data_ptr(), self.d.stream)
+
     def axpby(self, a, x, b, y):
         _lib.call("sles_axpby", self.d.plan.handle, 3, float(a), x.data_ptr(), float(b), y.data_ptr(), self.d.stream)
 
     # -- forward ------------------------------------------------------------------
     def _rk4_stages(self, U):
-        """Stage states Y_0..Y_3 of one RK4 step from U and the new state: one
+        """Stage states Y_0..Y_3 of one RK4 step from U, and the new state: one
         fused RK4 step (the forward pass's own kernels, so the recomputed
-        states equal the forward ones bit for bit) that also writes Y_1..Y_3."""
+        states equal the forward ones bit for bit) that also writes Y_1..Y_3.
+        U is overwritten with the new state (it is returned); Y_0 is the
+        step's own copy of U (written by the step as U_hat0), so no field is
+        copied."""
         d = self.d
-        d.U_hat.copy_(U)
-        d.U_hat1.copy_(U)
+        y0 = torch.empty_like(U)
         st = torch.empty((9,) + tuple(U.shape[1:]), dtype=U.dtype, device=U.device)
         prm = fused.params(d, dt=self.dt)
-        _lib.call("sles_rk4_stage_states", d.plan.handle, ctypes.byref(prm), d.U_hat.data_ptr(),
-                  d.U_hat0.data_ptr(), d.U_hat1.data_ptr(), st.data_ptr(), fused.work(d).data_ptr(), d.stream)
-        return [U, st[0:3], st[3:6], st[6:9]], d.U_hat.clone()
+        _lib.call("sles_rk4_stage_states", d.plan.handle, ctypes.byref(prm), U.data_ptr(),
+                  y0.data_ptr(), d.U_hat1.data_ptr(), st.data_ptr(), fused.work(d).data_ptr(), d.stream)
+        return [y0, st[0:3], st[3:6], st[6:9]], U
 
     def run_forward(self, U0, keep=None):
         """Advance nsteps; returns the final state and the checkpoints
-        {step: (U, F_prev)}.  For ETD2 the states of the last segment are kept
-        as well (entry "tail": [U_n, ...]): the backward sweep starts there
-        and needs no recomputation for it."""
+        {step: (U, F_prev)}.  With keep_last_segment, what the backward sweep
+        needs for the last segment is kept as well (entry "tail": ETD2 the
+        states [U_n, ...], RK4 the stage states [Y_0..Y_3] of every step,
+        written by the same fused step kernels): the sweep starts there and
+        needs no recomputation for it."""
         d = self.d
         ckpt = {}
         d.U_hat.copy_(U0)
@@ -118,11 +139,16 @@ class DifferentiableLES:
         for n in range(self.nsteps):
             if n % self.K == 0:
                 ckpt[n] = (d.U_hat.clone(), d.F0.clone() if (self.scheme == "ETD2" and n > 0) else None)
-            if self.scheme == "ETD2" and n >= last:
-                tail.append(ckpt[n][0] if n == last else d.U_hat.clone())
+            keep = self.keep_last_segment and n >= last
             if self.scheme == "RK4":
-                fused.rk4_step(d, physical=False)
+                if keep:
+                    Ys, _ = self._rk4_stages(d.U_hat)  # advances d.U_hat in place
+                    tail.append(Ys)
+                else:
+                    fused.rk4_step(d, physical=False)
             else:
+                if keep:
+                    tail.append(ckpt[n][0] if n == last else d.U_hat.clone())
                 fused.etd2_step(d, physical=False)
         if tail:
             ckpt["tail"] = (last, tail)
@@ -137,13 +163,17 @@ class DifferentiableLES:
         return self._backward_etd2(ckpt, lam)
 
     def _backward_rk4(self, ckpt, lam):
-        for start in sorted(ckpt, reverse=True):
+        tail_start, tail_stages = ckpt.get("tail", (None, None))
+        for start in sorted((k for k in ckpt if k != "tail"), reverse=True):
             end = min(start + self.K, self.nsteps)
-            U = ckpt[start][0].clone()
-            stages = []
-            for n in range(start, end):
-                Ys, U = self._rk4_stages(U)
-                stages.append(Ys)
+            if start == tail_start:
+                stages = tail_stages  # kept by the forward pass
+            else:
+                U = ckpt[start][0].clone()
+                stages = []
+                for n in range(start, end):
+                    Ys, U = self._rk4_stages(U)
+                    stages.append(Ys)
             for Ys in reversed(stages):
                 lam = self._rk4_step_adjoint(Ys, lam)
         return lam
@@ -188,7 +218,7 @@ class DifferentiableLES:
                     lam_next2, lam_next = lam_next, lam_out
                 else:
                     # step 0: RK4 bootstrap U_1 = RK4(U_0), plus F(U_0) in the step to U_2
-                    Ys, _ = self._rk4_stages(Un)
+                    Ys, _ = self._rk4_stages(Un.clone())
                     lam0 = self._rk4_step_adjoint(Ys, lam_next)
                     if lam_next2 is not None:
                         # U_2 = ... - w0 F(U_0): contribution J_F(U_0)^T (-w0 lam_2)
@@ -205,17 +235,19 @@ class DifferentiableLES:
                   out.data_ptr(), self.d.stream)
 
     def _rk4_step_adjoint(self, Ys, lam):
-        """lam_n from lam_{n+1} for one RK4 step with stage states Ys."""
-        lY = torch.empty_like(lam)
-        mu = torch.empty_like(lam)
+        """lam_n from lam_{n+1} for one RK4 step with stage states Ys: four
+        fused calls that form mu_s on the fly, then one summation pass."""
+        lY = [None] * 4
+        for s in (3, 2, 1, 0):
+            lY[s] = torch.empty_like(lam)
+            if s == 3:
+                # lY_3 = J_3^T (dt a_3 lam)
+                self.vjp_combined(Ys[3], self.dt * RK4_A[3], lam, 0.0, None, lY[3])
+            else:
+                # lY_s = J_s^T (dt b_s lY_{s+1} + dt a_s lam)
+                self.vjp_combined(Ys[s], self.dt * RK4_B[s], lY[s + 1], self.dt * RK4_A[s], lam, lY[s])
         acc = torch.empty_like(lam)
-        self.lincomb(self.dt * RK4_A[3], lam, 0.0, lam, mu)  # mu = dt a_3 lam
-        self.vjp(Ys[3], mu, lY)  # lY = J_3^T mu
-        self.lincomb(1.0, lam, 1.0, lY, acc)  # acc = lam + lY_3
-        for s in (2, 1, 0):
-            self.lincomb(self.dt * RK4_B[s], lY, self.dt * RK4_A[s], lam, mu)  # mu = dt b_s lY + dt a_s lam
-            self.vjp(Ys[s], mu, lY)
-            self.axpby(1.0, lY, 1.0, acc)
+        self.sum_fields([lam, lY[3], lY[2], lY[1], lY[0]], acc)  # lam + sum_s lY_s
         return acc
 
     def __call__(self, U0, Cs):

@@ -224,14 +224,15 @@ def differentiable_study(N=256, N_target=512, steps=50, target_steps=50, dt=2e-3
 
 
 def adjoint_cost(N=256, scheme="RK4", les="Smagorinsky", precision="fp32", steps=20, checkpoint_every=10,
-                 dt=1e-3, Cs=0.17, urms=0.25, seed=4):
+                 dt=1e-3, Cs=0.17, urms=0.25, seed=4, keep_last_segment=True):
     """Backward / forward time of a DifferentiableLES window (loss = sum of
     E(k), k <= N/3, so the adjoint seed is cheap): the cost of the adjoint step
-    relative to the forward step, with checkpointing."""
+    relative to the forward step, with checkpointing (keep_last_segment=False:
+    every segment recomputed in the backward sweep)."""
     from .adjoint import DifferentiableLES, shell_spectrum
 
     s = DifferentiableLES(N=N, nu=1e-4, dt=dt, nsteps=steps, les_model=les, scheme=scheme, precision=precision,
-                          checkpoint_every=checkpoint_every)
+                          checkpoint_every=checkpoint_every, keep_last_segment=keep_last_segment)
     cd = torch.complex64 if precision == "fp32" else torch.complex128
     U0 = isotropic_spectrum(N, seed=seed, urms=urms, dtype=cd)
 
@@ -251,7 +252,8 @@ def adjoint_cost(N=256, scheme="RK4", les="Smagorinsky", precision="fp32", steps
     once()
     f, bwd = once()
     res = {"N": N, "scheme": scheme, "les_model": les, "precision": precision, "steps": steps,
-           "checkpoint_every": checkpoint_every, "forward_s": f, "backward_s": bwd, "backward_over_forward": bwd / f}
+           "checkpoint_every": checkpoint_every, "keep_last_segment": keep_last_segment, "forward_s": f,
+           "backward_s": bwd, "backward_over_forward": bwd / f}
     print(json.dumps(res), flush=True)
     return res
 
@@ -348,6 +350,7 @@ def main(argv=None):
     rc.add_argument("--les", default="Smagorinsky")
     rc.add_argument("--precision", default="fp32")
     rc.add_argument("--steps", type=int, default=20)
+    rc.add_argument("--recompute-all", action="store_true", help="recompute the last segment too")
     ra = sub.add_parser("ensemble-adjoint")
     ra.add_argument("--steps", type=int, default=200)
     ra.add_argument("--out", default=None)
@@ -362,7 +365,7 @@ def main(argv=None):
         precision_tgv(a.n, a.steps)
         return
     if a.cmd == "adjoint-cost":
-        adjoint_cost(a.n, a.scheme, a.les, a.precision, a.steps)
+        adjoint_cost(a.n, a.scheme, a.les, a.precision, a.steps, keep_last_segment=not a.recompute_all)
         return
     if a.cmd == "ensemble-adjoint":
         res = ensemble_adjoint(steps=a.steps)

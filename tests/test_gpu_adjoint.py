@@ -225,7 +225,8 @@ def test_rk4_stage_states(les):
     s = _solver(N, les, dt=0.01, nu=0.02)
     s._set_cs(0.15)
     U = _state(N, 31)
-    Ys, new = s._rk4_stages(U)
+    Ys, new = s._rk4_stages(U.clone())
+    assert torch.equal(Ys[0], U)
     # reference stages from the RHS
     k = torch.empty_like(U)
     ref = [U]
@@ -239,3 +240,88 @@ def test_rk4_stage_states(les):
     d.U_hat1.copy_(U)
     fused.rk4_step(d, dt=s.dt, physical=False)
     assert torch.equal(d.U_hat, new)
+
+
+@pytest.mark.parametrize("les", [None, "Smagorinsky", "Vreman"])
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_vjp_combined(les, precision):
+    """sles_adjoint_rhs_combined against its composition from separate calls
+    (sles_lincomb then sles_adjoint_rhs), with and without z, the d/dCs
+    accumulation, and a plain sles_adjoint_rhs call afterwards (the combined
+    call stores its mu in the workspace; the plain call must not read it)."""
+    N = 24
+    s = _solver(N, les, precision=precision, forcing=0.05)
+    s._set_cs(0.16)
+    cd = s.d.cdtype
+    Y = _state(N, 5).to(cd)
+    g = torch.Generator(device="cuda").manual_seed(6)
+
+    def rnd():
+        return torch.complex(torch.randn(Y.shape, generator=g, device="cuda", dtype=torch.float64),
+                             torch.randn(Y.shape, generator=g, device="cuda", dtype=torch.float64)).to(cd)
+
+    x, z = rnd(), rnd()
+    a, b = 0.37, -1.25
+    tol = 1e-12 if precision == "fp64" else 2e-5
+    mu = torch.empty_like(Y)
+    ref = torch.empty_like(Y)
+    out = torch.empty_like(Y)
+    for zz, bb in ((z, b), (None, 0.0)):
+        s.lincomb(a, x, bb, zz if zz is not None else x, mu)
+        s.cs_grad.zero_()
+        s.vjp(Y, mu, ref)
+        cs_ref = float(s.cs_grad.item())
+        s.cs_grad.zero_()
+        s.vjp_combined(Y, a, x, bb, zz, out)
+        assert float((out - ref).abs().max()) <= tol * float(ref.abs().max())
+        if les:
+            assert abs(float(s.cs_grad.item()) - cs_ref) <= tol * abs(cs_ref)
+    # plain call after a combined one reads its own mu
+    s.vjp_combined(Y, a, x, b, z, out)
+    s.vjp(Y, x, out)
+    s.vjp(Y, x, ref)
+    assert torch.equal(out, ref)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_sum_fields(precision):
+    N = 16
+    s = _solver(N, None, precision=precision)
+    cd = s.d.cdtype
+    g = torch.Generator(device="cuda").manual_seed(8)
+    ts = [torch.complex(torch.randn((3, N, N, N // 2 + 1), generator=g, device="cuda", dtype=torch.float64),
+                        torch.randn((3, N, N, N // 2 + 1), generator=g, device="cuda", dtype=torch.float64)).to(cd)
+          for _ in range(5)]
+    out = torch.empty_like(ts[0])
+    s.sum_fields(ts, out)
+    ref = ts[0].clone()
+    for t in ts[1:]:
+        ref = ref + t
+    assert torch.equal(out, ref)
+    s.sum_fields(ts[:1], out)
+    assert torch.equal(out, ts[0])
+
+
+@pytest.mark.parametrize("scheme", ["RK4", "ETD2"])
+def test_keep_last_segment_is_bitwise_neutral(scheme):
+    """Keeping the last segment's states in the forward pass (no recomputation
+    for it) gives the same final state and gradients, bit for bit, as
+    recomputing every segment from its checkpoint."""
+    from sles.adjoint import DifferentiableLES, spectrum_loss
+
+    N, nsteps, K = 16, 7, 3
+    U0 = _state(N, 41)
+    target = TR.shell_spectrum(torch.as_tensor(_state(N, 42).cpu()), N, kmax=6).cuda()
+    res = []
+    for keep in (True, False):
+        s = DifferentiableLES(N=N, nu=0.01, dt=0.005, nsteps=nsteps, les_model="Vreman", scheme=scheme,
+                              precision="fp64", checkpoint_every=K, keep_last_segment=keep)
+        U = U0.clone().requires_grad_(True)
+        c = torch.tensor(0.1, dtype=torch.float64, device="cuda", requires_grad=True)
+        out = s(U, c)
+        loss = spectrum_loss(out, target, N, kmax=6)
+        loss.backward()
+        res.append((out.detach().clone(), U.grad.clone(), float(c.grad)))
+    assert torch.equal(res[0][0], res[1][0])
+    assert torch.equal(res[0][1], res[1][1])
+    assert res[0][2] == res[1][2]
