@@ -358,12 +358,12 @@ __global__ void __launch_bounds__(ZLines<PL>::THREADS)
   constexpr bool WV = WaveLocal<PL>::value;  // each line inside one wavefront: no block barriers
   using IFFT = LineFFTP<T, PL, +1, WV>;
   using FFT = LineFFTP<T, PL, -1, WV>;
-  __shared__ cpx<T> lds[LPB * LdsLine<T>::stride(N)];
+  __shared__ cpx<T> lds[LPB * LdsLine<T, N>::STRIDE];
   const int line = threadIdx.x / NT, t = threadIdx.x % NT;
   const long row = (long)blockIdx.x * LPB + line;
   const bool valid = row < (long)N * N;
   const long base = (valid ? row : 0) * NH;
-  const LdsLine<T> map{line * LdsLine<T>::stride(N)};
+  const LdsLine<T, N> map{line * LdsLine<T, N>::STRIDE};
   const TwRegs<T, PL> twr(tw, t);  // read once, used by all 17 transforms
   cpx<T>* w[17];
 #pragma unroll
@@ -510,7 +510,7 @@ __global__ void __launch_bounds__(ZLines<PL>::THREADS)
 #define SLES_FWD(QA, QB)                                               \
   _Pragma("unroll") for (int m = 0; m < P; ++m) v[m] = {QA[m], QB[m]}; \
   FFT::run(v, lds, map, t, twr);                                        \
-  split_pair<T, N, P, NT, WV>(v, lds, map, t, Ah, Bh);
+  split_pair<T, N, P, NT, WV>(v, lds, map.plain(), t, Ah, Bh);
   SLES_FWD(a[2][0], a[2][1])
 #pragma unroll
   for (int m = 0; m < KH; ++m) {

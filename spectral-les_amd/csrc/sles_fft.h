@@ -216,15 +216,40 @@ struct Butterfly<T, 8, DIR> {
 // ---------------------------------------------------------------------------
 // LDS index maps used for the exchange between stages.
 // ---------------------------------------------------------------------------
-// Lines stored contiguously (contiguous-axis passes): position p of the line,
-// with one pad element after every 256 bytes so that the stride-R writes of
-// an exchange spread over all LDS banks.  A line occupies stride(N) elements.
-template <typename T>
-struct LdsLine {
-  static constexpr int S = sizeof(T) == 8 ? 4 : 5;  // 16 (FP64) / 32 (FP32) complex per 256 B
-  static constexpr int stride(int n) { return n + (n >> S); }
+// Lines stored contiguously (contiguous-axis passes): position p of the line.
+// LDS serves 128 bytes per clock from 32 four-byte banks, so E = 8 (FP64) /
+// 16 (FP32) complex elements share one bank row.  When E divides N, the
+// elements of each row are permuted by an XOR with g(row): p ^ g(p / E),
+//   g(b) = (b ^ 2b) mod 8                    (FP64)
+//   g(b) = (b ^ 2b ^ 4b) mod 16              (FP32)
+// which keeps every access group of an exchange (stride-R writes, the
+// R-blocks of the second stage, contiguous reads) on distinct banks for the
+// radix-4 and radix-8 plans of 64..512 points.  Other N keep the older
+// layout: one pad element after every 256 bytes.  A line occupies STRIDE
+// elements.  plain() is the identity layout of the same region, for the
+// two-for-one split, whose mirrored reads (N - k) are conflict-free only
+// without a permutation.
+struct LdsPlain {
   int base;
-  __device__ __forceinline__ int operator()(int p) const { return base + p + (p >> S); }
+  __device__ __forceinline__ int operator()(int p) const { return base + p; }
+};
+template <typename T, int N>
+struct LdsLine {
+  static constexpr int LOGE = sizeof(T) == 8 ? 3 : 4;
+  static constexpr int E = 1 << LOGE;
+  static constexpr bool SWZ = N % E == 0;
+  static constexpr int STRIDE = SWZ ? N : N + (N >> (LOGE + 1));
+  int base;
+  __device__ __forceinline__ int operator()(int p) const {
+    if constexpr (SWZ) {
+      const int b = p >> LOGE;
+      const int g = (sizeof(T) == 8 ? (b ^ (b << 1)) : (b ^ (b << 1) ^ (b << 2))) & (E - 1);
+      return base + (p ^ g);
+    } else {
+      return base + p + (p >> (LOGE + 1));
+    }
+  }
+  __device__ __forceinline__ LdsPlain plain() const { return {base}; }
 };
 // Columns interleaved (strided passes): position p of column c, TK columns.
 template <int TK>

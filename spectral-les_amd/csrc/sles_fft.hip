@@ -43,8 +43,8 @@ __global__ void __launch_bounds__(ZShape<N>::THREADS)
             const cpx<T>* __restrict__ tw) {
   using S = ZShape<N>;
   constexpr int P = S::P, NT = S::NT, NH = S::NH;
-  __shared__ cpx<T> lds[S::LPB * LdsLine<T>::stride(N)];
-  const LdsLine<T> map{(int)(threadIdx.x / S::NT) * LdsLine<T>::stride(N)};
+  __shared__ cpx<T> lds[S::LPB * LdsLine<T, N>::STRIDE];
+  const LdsLine<T, N> map{(int)(threadIdx.x / S::NT) * LdsLine<T, N>::STRIDE};
   const int line = threadIdx.x / NT, t = threadIdx.x % NT;
   const TwRegs<T, LinePlan<N>> twr(tw, t);  // issued together with the data loads
   const long ra = 2 * ((long)blockIdx.x * S::LPB + line), rb = ra + 1;
@@ -59,16 +59,18 @@ __global__ void __launch_bounds__(ZShape<N>::THREADS)
     v[m].y = vb ? xb : T(0);
   }
   LineFFTP<T, LinePlan<N>, -1, WaveLocal<LinePlan<N>>::value>::run(v, lds, map, t, twr);
-  // Split Z = FFT(a + i b):  A_k = (Z_k + conj Z_{N-k})/2,  B_k = (Z_k - conj Z_{N-k})/(2i)
+  // Split Z = FFT(a + i b):  A_k = (Z_k + conj Z_{N-k})/2,  B_k = (Z_k - conj Z_{N-k})/(2i),
+  // through the unpermuted layout of the line (mirrored reads)
+  const LdsPlain pmap = map.plain();
 #pragma unroll
-  for (int m = 0; m < P; ++m) lds[map(t + m * NT)] = v[m];
+  for (int m = 0; m < P; ++m) lds[pmap(t + m * NT)] = v[m];
   line_sync<WaveLocal<LinePlan<N>>::value>();
 #pragma unroll
   for (int m = 0; m <= P / 2; ++m) {
     const int k = t + m * NT;
     if (k <= N / 2) {
       const cpx<T> zk = v[m];
-      const cpx<T> zn = conj(lds[map((N - k) % N)]);
+      const cpx<T> zn = conj(lds[pmap((N - k) % N)]);
       const cpx<T> a = T(0.5) * (zk + zn);
       const cpx<T> d = zk - zn;
       const cpx<T> b = {T(0.5) * d.y, T(-0.5) * d.x};
@@ -84,8 +86,8 @@ __global__ void __launch_bounds__(ZShape<N>::THREADS)
             const cpx<T>* __restrict__ tw) {
   using S = ZShape<N>;
   constexpr int P = S::P, NT = S::NT, NH = S::NH;
-  __shared__ cpx<T> lds[S::LPB * LdsLine<T>::stride(N)];
-  const LdsLine<T> map{(int)(threadIdx.x / S::NT) * LdsLine<T>::stride(N)};
+  __shared__ cpx<T> lds[S::LPB * LdsLine<T, N>::STRIDE];
+  const LdsLine<T, N> map{(int)(threadIdx.x / S::NT) * LdsLine<T, N>::STRIDE};
   const int line = threadIdx.x / NT, t = threadIdx.x % NT;
   const TwRegs<T, LinePlan<N>> twr(tw, t);  // issued together with the data loads
   const long ra = 2 * ((long)blockIdx.x * S::LPB + line), rb = ra + 1;

@@ -246,7 +246,8 @@ __device__ __forceinline__ void pair_finish(cpx<T> (&v)[P], const PairRaw<T, P>&
 
 // After a forward FFT of a + i b held in v (natural layout), split into the
 // half-spectra A_k, B_k of a and b for this thread's bins k = t + m*NT <= N/2,
-// m = 0..P/2.  Uses the line's LDS row; ends with a barrier.
+// m = 0..P/2.  Uses the line's LDS row through an unpermuted map (the
+// line's LdsLine::plain()); ends with a barrier.
 template <typename T, int N, int P, int NT, bool WAVE = false, class Map>
 __device__ __forceinline__ void split_pair(const cpx<T> (&v)[P], cpx<T>* lds, const Map& map, int t,
                                            cpx<T> (&A)[P / 2 + 1], cpx<T> (&B)[P / 2 + 1]) {
@@ -295,12 +296,12 @@ __global__ void __launch_bounds__(ZLines<PL>::THREADS)
   constexpr bool WV = WaveLocal<PL>::value;  // each line inside one wavefront: no block barriers
   using IFFT = LineFFTP<T, PL, +1, WV>;
   using FFT = LineFFTP<T, PL, -1, WV>;
-  __shared__ cpx<T> lds[LPB * LdsLine<T>::stride(N)];
+  __shared__ cpx<T> lds[LPB * LdsLine<T, N>::STRIDE];
   const int line = threadIdx.x / NT, t = threadIdx.x % NT;
   const long row = (long)blockIdx.x * LPB + line;
   const bool valid = row < (long)N * N;
   const unsigned row0 = (unsigned)((valid ? row : 0) * NH);
-  const LdsLine<T> map{line * LdsLine<T>::stride(N)};
+  const LdsLine<T, N> map{line * LdsLine<T, N>::STRIDE};
   const TwRegs<T, PL> twr(tw, t);  // read once, used by all 11 transforms
   cpx<T>* w[9];
 #pragma unroll
@@ -371,7 +372,7 @@ __global__ void __launch_bounds__(ZLines<PL>::THREADS)
 #define SLES_FWD(QA, QB)                                     \
   _Pragma("unroll") for (int m = 0; m < P; ++m) v[m] = {QA[m], QB[m]}; \
   FFT::run(v, lds, map, t, twr);                             \
-  split_pair<T, N, P, NT, WV>(v, lds, map, t, A, B);
+  split_pair<T, N, P, NT, WV>(v, lds, map.plain(), t, A, B);
   SLES_FWD(q4, q5)  // tau_xy, tau_xz
 #pragma unroll
   for (int m = 0; m < KH; ++m) {
@@ -423,12 +424,12 @@ __global__ void __launch_bounds__(ZLines<PL>::THREADS)
   constexpr bool WV = WaveLocal<PL>::value;  // each line inside one wavefront: no block barriers
   using IFFT = LineFFTP<T, PL, +1, WV>;
   using FFT = LineFFTP<T, PL, -1, WV>;
-  __shared__ cpx<T> lds[LPB * LdsLine<T>::stride(N)];
+  __shared__ cpx<T> lds[LPB * LdsLine<T, N>::STRIDE];
   const int line = threadIdx.x / NT, t = threadIdx.x % NT;
   const long row = (long)blockIdx.x * LPB + line;
   const bool valid = row < (long)N * N;
   const unsigned row0 = (unsigned)((valid ? row : 0) * NH);
-  const LdsLine<T> map{line * LdsLine<T>::stride(N)};
+  const LdsLine<T, N> map{line * LdsLine<T, N>::STRIDE};
   const TwRegs<T, PL> twr(tw, t);  // read once, used by all 5 transforms
   cpx<T>* w[6];
 #pragma unroll
@@ -464,7 +465,7 @@ __global__ void __launch_bounds__(ZLines<PL>::THREADS)
 #pragma unroll
   for (int m = 0; m < P; ++m) v[m] = {n0[m], n1[m]};
   FFT::run(v, lds, map, t, twr);
-  split_pair<T, N, P, NT, WV>(v, lds, map, t, A, B);
+  split_pair<T, N, P, NT, WV>(v, lds, map.plain(), t, A, B);
 #pragma unroll
   for (int m = 0; m < KH; ++m)
     if (valid && t + m * NT <= N / 2) {
@@ -474,7 +475,7 @@ __global__ void __launch_bounds__(ZLines<PL>::THREADS)
 #pragma unroll
   for (int m = 0; m < P; ++m) v[m] = {n2[m], T(0)};
   FFT::run(v, lds, map, t, twr);
-  split_pair<T, N, P, NT, WV>(v, lds, map, t, A, B);
+  split_pair<T, N, P, NT, WV>(v, lds, map.plain(), t, A, B);
 #pragma unroll
   for (int m = 0; m < KH; ++m)
     if (valid && t + m * NT <= N / 2) st32(w[2], row0 + t + m * NT, A[m]);
