@@ -662,14 +662,19 @@ __device__ __forceinline__ void epilogue(cpx<T> (&s)[3], eidx e, int kx, int ky,
   }
 }
 
-// Blocks of at most 512 threads for every plan: the three accumulated
-// components, the transform and the epilogue need ~180 VGPRs per thread with
-// the radix-4 FP64 plan, which a 1024-thread block (128 VGPRs) would spill.
+// Blocks of at most 512 threads, except FP32 radix-4 (1024): the three
+// accumulated components, the transform and the epilogue need ~200 VGPRs per
+// thread with the radix-4 FP64 plan, which a 1024-thread block (128 VGPRs)
+// would spill; the FP32 radix-4 form needs ~110 and gains 128-byte row
+// segments per tile from the wider block (256^3 -5 %, 512^3 -10 %).
+template <typename T, int N, class PL>
+using XShape = StridedShape<T, N, PL, (sizeof(T) == 4 && PL::P == 4) ? 1024 : 512>;
+
 template <typename T, int N, int LES, class PL>
-__global__ void __launch_bounds__((StridedShape<T, N, PL, 512>::THREADS))
+__global__ void __launch_bounds__((XShape<T, N, PL>::THREADS))
     k_xfwd(cpx<T>* __restrict__ W, long C, cpx<T>* uh, cpx<T>* uh0, cpx<T>* uh1, cpx<T>* aux, StepArgs a,
            const cpx<T>* __restrict__ tw, int xcd) {
-  using S = StridedShape<T, N, PL, 512>;
+  using S = XShape<T, N, PL>;
   constexpr int P = S::P, NT = S::NT, TK = S::TK, NH = N / 2 + 1;
   constexpr int PPB = N * TK;  // ping-pong exchange buffers
   using FFT = LineFFTP<T, PL, -1, false, PPB>;
@@ -804,7 +809,7 @@ struct ProfScope {
 template <typename T, int N, class PS, class PX, class PZ>
 struct Fused {
   using SS = WideShape<T, N, PS>;  // x/y inverse and y forward
-  using SX = StridedShape<T, N, PX, 512>;  // x forward + stage update
+  using SX = XShape<T, N, PX>;  // x forward + stage update
   static constexpr int ZB = (int)(((long)N * N + ZLines<PZ>::LPB - 1) / ZLines<PZ>::LPB);
 
   static int stage(const sles_plan_s* p, int les, T coeff, cpx<T>* uh, cpx<T>* W, double* part, hipStream_t s) {
