@@ -356,9 +356,9 @@ __global__ void __launch_bounds__(ZLines<PL>::THREADS)
   constexpr int P = PL::P, NT = PL::NT, NH = N / 2 + 1, KH = P / 2 + 1;
   constexpr int LPB = ZLines<PL>::LPB;
   constexpr bool WV = WaveLocal<PL>::value;  // each line inside one wavefront: no block barriers
-  using IFFT = LineFFTP<T, PL, +1, WV>;
-  using FFT = LineFFTP<T, PL, -1, WV>;
-  __shared__ cpx<T> lds[LPB * LdsLine<T, N>::STRIDE];
+  using IFFT = LineFFTP<T, PL, +1, WV, ZPPB<T, N, PL>>;
+  using FFT = LineFFTP<T, PL, -1, WV, ZPPB<T, N, PL>>;
+  __shared__ cpx<T> lds[LPB * LdsLine<T, N>::STRIDE + ZPPB<T, N, PL>];
   const int line = threadIdx.x / NT, t = threadIdx.x % NT;
   const long row = (long)blockIdx.x * LPB + line;
   const bool valid = row < (long)N * N;

@@ -197,12 +197,19 @@ __global__ void __launch_bounds__((WideShape<T, N, PL>::THREADS))
 // ---------------------------------------------------------------------------
 // 3. z kernel: one line per (x, y) row.
 // ---------------------------------------------------------------------------
+// Lines of up to 64 threads (one wavefront or less): 256-thread blocks of
+// LPB lines with wavefront-local exchanges.  Longer lines (radix-4 at 512
+// points: 128 threads) need block barriers: one line per block, so a barrier
+// waits for that line's two wavefronts only, and ping-pong exchange buffers
+// (ZPPB), one barrier per exchange (512^3 FP64 z pass 6.28 -> 5.24 ms).
 template <class PL>
 struct ZLines {
   static constexpr int NT = PL::NT;
-  static constexpr int LPB = (256 / NT) > 0 ? (256 / NT) : 1;
+  static constexpr int LPB = NT > 64 ? 1 : ((256 / NT) > 0 ? (256 / NT) : 1);
   static constexpr int THREADS = LPB * NT;
 };
+template <typename T, int N, class PL>
+constexpr int ZPPB = WaveLocal<PL>::value ? 0 : ZLines<PL>::LPB * LdsLine<T, N>::STRIDE;
 
 // Line pairs Z_k = A_k + i B_k of two half-spectra A, B (uniform field bases,
 // line starting at element row0): pair_issue loads bins kk = min(k, N-k),
@@ -294,9 +301,9 @@ __global__ void __launch_bounds__(ZLines<PL>::THREADS)
   constexpr int P = PL::P, NT = PL::NT, NH = N / 2 + 1, KH = P / 2 + 1;
   constexpr int LPB = ZLines<PL>::LPB;
   constexpr bool WV = WaveLocal<PL>::value;  // each line inside one wavefront: no block barriers
-  using IFFT = LineFFTP<T, PL, +1, WV>;
-  using FFT = LineFFTP<T, PL, -1, WV>;
-  __shared__ cpx<T> lds[LPB * LdsLine<T, N>::STRIDE];
+  using IFFT = LineFFTP<T, PL, +1, WV, ZPPB<T, N, PL>>;
+  using FFT = LineFFTP<T, PL, -1, WV, ZPPB<T, N, PL>>;
+  __shared__ cpx<T> lds[LPB * LdsLine<T, N>::STRIDE + ZPPB<T, N, PL>];
   const int line = threadIdx.x / NT, t = threadIdx.x % NT;
   const long row = (long)blockIdx.x * LPB + line;
   const bool valid = row < (long)N * N;
@@ -422,9 +429,9 @@ __global__ void __launch_bounds__(ZLines<PL>::THREADS)
   constexpr int P = PL::P, NT = PL::NT, NH = N / 2 + 1, KH = P / 2 + 1;
   constexpr int LPB = ZLines<PL>::LPB;
   constexpr bool WV = WaveLocal<PL>::value;  // each line inside one wavefront: no block barriers
-  using IFFT = LineFFTP<T, PL, +1, WV>;
-  using FFT = LineFFTP<T, PL, -1, WV>;
-  __shared__ cpx<T> lds[LPB * LdsLine<T, N>::STRIDE];
+  using IFFT = LineFFTP<T, PL, +1, WV, ZPPB<T, N, PL>>;
+  using FFT = LineFFTP<T, PL, -1, WV, ZPPB<T, N, PL>>;
+  __shared__ cpx<T> lds[LPB * LdsLine<T, N>::STRIDE + ZPPB<T, N, PL>];
   const int line = threadIdx.x / NT, t = threadIdx.x % NT;
   const long row = (long)blockIdx.x * LPB + line;
   const bool valid = row < (long)N * N;
