@@ -145,3 +145,33 @@ def test_256_fp64_matches_oracle():
         O.rk4_step(o)
     assert rel(g.U_hat, o.U_hat) < 1e-10
     assert rel(torch.stack(g.U), np.array(o.U)) < 1e-10
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_512_z_plans_agree(precision):
+    """At 512 points the radix-4 z lines span two wavefronts (one line per
+    block, ping-pong exchange buffers, block barriers) while the radix-8 lines
+    fit one wavefront: one Smagorinsky RHS at 512^3 through either z plan
+    agrees to rounding (FP64 1e-12, FP32 1e-5 relative)."""
+    from sles import fused
+    from sles.initialize_domain import InitializeDomain
+
+    N = 512
+    d = InitializeDomain(N=N, nu=1e-4, method="RK4", LES_ON=1, LES_model="Smagorinsky", Cs=0.17,
+                         time_stepSize=1e-3, precision=precision, tuned=False)
+    g = torch.Generator(device="cuda").manual_seed(11)
+    shape = (3, N, N, N // 2 + 1)
+    d.U_hat.copy_(torch.complex(torch.randn(shape, generator=g, device="cuda", dtype=d.rdtype),
+                                torch.randn(shape, generator=g, device="cuda", dtype=d.rdtype)))
+    out4 = torch.empty_like(d.U_hat)
+    out8 = torch.empty_like(d.U_hat)
+    d.plan.set_tuning(z_radix4=1)
+    fused.rhs(d, out4)
+    d.plan.set_tuning(z_radix4=0)
+    fused.rhs(d, out8)
+    torch.cuda.synchronize()
+    tol = 1e-12 if precision == "fp64" else 1e-5
+    assert torch.isfinite(out4).all()
+    num = sum(float((out4[i] - out8[i]).abs().double().pow(2).sum()) for i in range(3))
+    den = sum(float(out8[i].abs().double().pow(2).sum()) for i in range(3))
+    assert (num / den) ** 0.5 < tol
