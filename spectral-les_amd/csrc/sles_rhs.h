@@ -113,10 +113,10 @@ __global__ void __launch_bounds__((WideShape<T, N, PL>::THREADS))
         for (int m = 0; m < P; ++m) stx(W + (3 + j) * C, at(m), w[m]);
     }
   } else {
-    // U_hat and omega_hat = 1j * (K x U_hat) (math_formula.py:22-24) from one
-    // read of the three components: the curl components first (they need all
-    // three), then each component transformed in place, so the live set stays
-    // at four P-point arrays
+    // U_hat and the x-derivatives of its y and z components, from one read
+    // of the three components: W0-2 = F_x^-1 U_j, W3 = F_x^-1 (1j Kx U_1),
+    // W4 = F_x^-1 (1j Kx U_2).  The y pass completes omega_hat = 1j (K x U_hat)
+    // (math_formula.py:22-24) from these five fields.
     cpx<T> u0[P], u1[P], u2[P];
 #pragma unroll
     for (int m = 0; m < P; ++m) {
@@ -132,14 +132,11 @@ __global__ void __launch_bounds__((WideShape<T, N, PL>::THREADS))
         for (int m = 0; m < P; ++m) stx(W + j * C, at(m), x[m]);
     };
 #pragma unroll
-    for (int m = 0; m < P; ++m) v[m] = mul_i(ky * u2[m] - kz * u1[m]);
+    for (int m = 0; m < P; ++m) v[m] = mul_i(kx(m) * u1[m]);
     fft_store(v, 3);
 #pragma unroll
-    for (int m = 0; m < P; ++m) v[m] = mul_i(kz * u0[m] - kx(m) * u2[m]);
+    for (int m = 0; m < P; ++m) v[m] = mul_i(kx(m) * u2[m]);
     fft_store(v, 4);
-#pragma unroll
-    for (int m = 0; m < P; ++m) v[m] = mul_i(kx(m) * u1[m] - ky * u0[m]);
-    fft_store(v, 5);
     fft_store(u0, 0);
     fft_store(u1, 1);
     fft_store(u2, 2);
@@ -183,14 +180,50 @@ __global__ void __launch_bounds__((WideShape<T, N, PL>::THREADS))
         for (int m = 0; m < P; ++m) stx(W + (6 + j) * C, at(m), w[m]);
     }
   }
+  if constexpr (LES) {
 #pragma unroll 1
-  for (int j = LES ? 3 : 0; j < 6; ++j) {
+    for (int j = 3; j < 6; ++j) {
 #pragma unroll
-    for (int m = 0; m < P; ++m) v[m] = ldx(W + j * C, at(m));
-    FFT::run(v, lds, map, t, twr);
-    if (col.valid)
+      for (int m = 0; m < P; ++m) v[m] = ldx(W + j * C, at(m));
+      FFT::run(v, lds, map, t, twr);
+      if (col.valid)
 #pragma unroll
-      for (int m = 0; m < P; ++m) stx(W + j * C, at(m), v[m]);
+        for (int m = 0; m < P; ++m) stx(W + j * C, at(m), v[m]);
+    }
+  } else {
+    // from W0-2 = U_j, W3 = Dx U_1, W4 = Dx U_2 (x already inverse-transformed):
+    // omega = 1j (K x U) = (1j (Ky w - Kz v), 1j Kz u - Dx w, Dx v - 1j Ky u),
+    // then every field y-transformed: W0-2 = U, W3-5 = omega
+    auto ky = [&](int m) { return (T)kfreq(t + m * NT, N); };
+    const T kz = (T)iz;
+    auto fft_store = [&](cpx<T>(&x)[P], int j) {
+      FFT::run(x, lds, map, t, twr);
+      if (col.valid)
+#pragma unroll
+        for (int m = 0; m < P; ++m) stx(W + j * C, at(m), x[m]);
+    };
+    cpx<T> u[P], w[P];
+#pragma unroll
+    for (int m = 0; m < P; ++m) {
+      u[m] = ldx(W, at(m));
+      v[m] = ldx(W + 3 * C, at(m));  // Dx v
+    }
+#pragma unroll
+    for (int m = 0; m < P; ++m) v[m] = v[m] - mul_i(ky(m) * u[m]);  // omega_z
+    fft_store(v, 5);
+#pragma unroll
+    for (int m = 0; m < P; ++m) v[m] = mul_i(kz * u[m]) - ldx(W + 4 * C, at(m));  // omega_y
+    fft_store(v, 4);
+    fft_store(u, 0);
+#pragma unroll
+    for (int m = 0; m < P; ++m) {
+      u[m] = ldx(W + C, at(m));      // v
+      w[m] = ldx(W + 2 * C, at(m));  // w
+      v[m] = mul_i(ky(m) * w[m] - kz * u[m]);  // omega_x
+    }
+    fft_store(v, 3);
+    fft_store(u, 1);
+    fft_store(w, 2);
   }
 }
 
