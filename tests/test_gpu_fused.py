@@ -47,7 +47,7 @@ def test_fused_rhs(les, N, forcing):
 
 
 @pytest.mark.parametrize("les", LES)
-@pytest.mark.parametrize("N", [16, 24, 48])
+@pytest.mark.parametrize("N", [16, 24, 48, 64, 96])
 def test_fused_rk4_steps(les, N):
     import sles.RK4_method as R
 
