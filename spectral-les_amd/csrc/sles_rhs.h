@@ -8,8 +8,8 @@
 // touch HBM.
 //
 //   LES (Smagorinsky / Vreman)              no LES
-//   1 x-inverse  U_hat(3) -> W0-2 = u, W3-5 = i Kx u       W0-2 = u, W3-5 = i K x u
-//   2 y-inverse  W0-5 -> + W6-8 = i Ky u (9 fields)        6 fields in place
+//   1 x-inverse  U_hat(3) -> W0-2 = u, W3-5 = i Kx u       W0-2 = u, W3-4 = i Kx v, i Kx w
+//   2 y-inverse  W0-5 -> + W6-8 = i Ky u (9 fields)        5 fields -> W0-2 = u, W3-5 = i K x u
 //   3 z kernel   9 half-spectra -> c2r of u, du/dx, du/dy, du/dz (12 fields)
 //                -> omega, u x omega, nu_R, tau = nu_R S, max(u)
 //                -> r2c -> W0-2 = F(u x omega)_i + 2i Kz tau_iz,
