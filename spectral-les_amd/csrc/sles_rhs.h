@@ -230,15 +230,16 @@ __global__ void __launch_bounds__((WideShape<T, N, PL>::THREADS))
 // ---------------------------------------------------------------------------
 // 3. z kernel: one line per (x, y) row.
 // ---------------------------------------------------------------------------
-// Lines of up to 64 threads (one wavefront or less): 256-thread blocks of
-// LPB lines with wavefront-local exchanges.  Longer lines (radix-4 at 512
+// Lines of up to 64 threads (one wavefront or less): single-wavefront blocks
+// of LPB = 64 / NT lines with wavefront-local exchanges (4 % faster than
+// 256-thread blocks at 256^3 FP64, 3 % FP32).  Longer lines (radix-4 at 512
 // points: 128 threads) need block barriers: one line per block, so a barrier
 // waits for that line's two wavefronts only, and ping-pong exchange buffers
 // (ZPPB), one barrier per exchange (512^3 FP64 z pass 6.28 -> 5.24 ms).
 template <class PL>
 struct ZLines {
   static constexpr int NT = PL::NT;
-  static constexpr int LPB = NT > 64 ? 1 : ((256 / NT) > 0 ? (256 / NT) : 1);
+  static constexpr int LPB = NT > 64 ? 1 : ((64 / NT) > 0 ? (64 / NT) : 1);
   static constexpr int THREADS = LPB * NT;
 };
 template <typename T, int N, class PL>
