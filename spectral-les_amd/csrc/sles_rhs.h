@@ -782,6 +782,25 @@ __global__ void __launch_bounds__((XShape<T, N, PL>::THREADS))
   }
 }
 
+// First level of the CFL reduction: block b folds part[b*chunk, (b+1)*chunk)
+// (NaN-propagating max) into out[b].
+static __global__ void k_nanmax_chunks(const double* part, int np, int chunk, double* out) {
+  double m = -INFINITY;
+  const int lo = blockIdx.x * chunk, hi = lo + chunk < np ? lo + chunk : np;
+  for (int i = lo + threadIdx.x; i < hi; i += blockDim.x) m = (isnan(m) || m > part[i]) ? m : part[i];
+  __shared__ double s[256];
+  s[threadIdx.x] = m;
+  block_sync();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) {
+      const double a = s[threadIdx.x], b = s[threadIdx.x + o];
+      s[threadIdx.x] = (isnan(a) || a > b) ? a : b;
+    }
+    block_sync();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = s[0];
+}
+
 // dt = min(CFL dx / max(u), 0.1 dx^2 / nu)   (src/SM_solver.py:245-272)
 static __global__ void k_cfl(const double* part, int np, double cfl, double dx, double nu, double* scalars) {
   double m = -INFINITY;
@@ -957,8 +976,14 @@ int step_impl(const sles_plan_s* p, int scheme, const sles_step_params* prm, voi
     double* pp = (ddt && st == 0) ? part : nullptr;
     if (F::stage(p, les, coeff, uh, W, pp, s)) return -1;
     if (pp) {
+      // two levels: up to 64 chunks of the per-block maxima, then one block
+      // (the partials area holds N^2 + 64 doubles; the z grid is at most N^2)
       ProfScope ps_(PK_CFL, s);
-      hipLaunchKernelGGL(k_cfl, dim3(1), dim3(1024), 0, s, (const double*)part, F::ZB, prm->cfl, prm->dx, prm->nu,
+      constexpr int NB = F::ZB < 64 ? F::ZB : 64;
+      constexpr int CHUNK = (F::ZB + NB - 1) / NB;
+      double* part2 = part + F::ZB;
+      hipLaunchKernelGGL(k_nanmax_chunks, dim3(NB), dim3(256), 0, s, (const double*)part, F::ZB, CHUNK, part2);
+      hipLaunchKernelGGL(k_cfl, dim3(1), dim3(64), 0, s, (const double*)part2, NB, prm->cfl, prm->dx, prm->nu,
                          scalars);
       if (check_launch("cfl")) return -1;
     }
