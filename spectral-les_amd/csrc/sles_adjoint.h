@@ -88,6 +88,21 @@ struct MuSrc {
   }
 };
 
+// out[b] = sum of part[b*chunk, (b+1)*chunk)  (first level, fixed order)
+static __global__ void k_sum_chunks(const double* part, int np, int chunk, double* out) {
+  __shared__ double sh[256];
+  double v = 0.0;
+  const int lo = blockIdx.x * chunk, hi = lo + chunk < np ? lo + chunk : np;
+  for (int i = lo + threadIdx.x; i < hi; i += blockDim.x) v += part[i];
+  sh[threadIdx.x] = v;
+  block_sync();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    block_sync();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = sh[0];
+}
+
 // acc[0] += sum of part[0..np)  (one block, fixed order)
 static __global__ void k_sum_add(const double* part, int np, double* acc) {
   __shared__ double sh[1024];
@@ -698,7 +713,14 @@ int adjoint_impl(const sles_plan_s* p, const sles_step_params* prm, const void* 
                          T(0), T(0), (double*)nullptr, tw);
       cp = nullptr;
   }
-  if (cp) hipLaunchKernelGGL(k_sum_add, dim3(1), dim3(1024), 0, s, (const double*)part, ZB, cs_grad);
+  if (cp) {
+    // two levels (up to 64 chunks, then one block); the partials area holds
+    // N^2 + 64 doubles and the z grid has at most N^2 blocks
+    constexpr int NB = ZB < 64 ? ZB : 64;
+    constexpr int CHUNK = (ZB + NB - 1) / NB;
+    hipLaunchKernelGGL(k_sum_chunks, dim3(NB), dim3(256), 0, s, (const double*)part, ZB, CHUNK, part + ZB);
+    hipLaunchKernelGGL(k_sum_add, dim3(1), dim3(64), 0, s, (const double*)(part + ZB), NB, cs_grad);
+  }
   hipLaunchKernelGGL((k_adj_yfwd<T, N, PS>), gs, bs, 0, s, A, C, tw, xcd);
   AdjArgs aa{prm->nu, prm->forcing, linear, accumulate};
   if (src.etd2)
