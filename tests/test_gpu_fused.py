@@ -85,10 +85,13 @@ def test_fused_etd2_cox_matthews(les):
 
 
 @pytest.mark.parametrize("les", [None, "Smagorinsky"])
-def test_fused_device_dt_matches_cfl_rule(les):
+@pytest.mark.parametrize("N", [16, 32, 64, 128])
+def test_fused_device_dt_matches_cfl_rule(les, N):
+    """The device CFL dt (per-block maxima of the z pass, reduced in two
+    levels: fewer z blocks than chunks at 16^3, one per chunk at 32^3,
+    several per chunk above) equals the reference rule on the same field."""
     from sles import fused
 
-    N = 32
     g, o, u = twins(N, seed=8, fused=True, **_kw(les, nu=0.002, method="RK4", CFL=0.5))
     umax = O.find_max_velocity(o.U)
     dt = O.compute_dt(o.U, umax, o.nu, dx=o.dx, CFL=0.5)
