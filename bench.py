@@ -69,13 +69,18 @@ def kernel_traffic_fields(les):
     }
 
 
-def xfwd_mean_fields(les):
+def xfwd_mean_fields(les, fused=False):
     """Mean (read + write) complex fields of the x-forward/update pass over the
     4 RK4 stages: stage 0 reads W + U_hat and writes U_hat0, U_hat1, U_hat;
     stages 1-2 read W + U_hat, U_hat0, U_hat1 and write U_hat1, U_hat;
-    stage 3 reads W + U_hat, U_hat1 and writes U_hat1, U_hat."""
+    stage 3 reads W + U_hat, U_hat1 and writes U_hat1, U_hat.
+    fused (FP32 form): stages 0-2 also write the next stage's x-inverse
+    output (6 fields with LES, 5 without), which then reads nothing more."""
     w = 6 if les else 3
     per_stage = [w + 3 + 9, w + 9 + 6, w + 9 + 6, w + 6 + 6]
+    if fused:
+        xi = 6 if les else 5
+        per_stage = [f + xi for f in per_stage[:3]] + per_stage[3:]
     return sum(per_stage) / 4.0
 
 
@@ -280,10 +285,10 @@ def roofline_of(res, N, precision, les=True):
     best = max(fields, key=lambda k: kern[k][0])
     tot_ms, cnt = kern[best]
     avg_s = tot_ms / cnt / 1e3
+    # the FP32 form fuses the next RK4 stage's x-inverse into the x-forward
+    fused = precision == "fp32"
     if best == "x_forward_update":
-        nbytes = xfwd_mean_fields(les) * C
-    elif best == "z_fused":
-        nbytes = sum(fields[best]) * C
+        nbytes = xfwd_mean_fields(les, fused) * C
     else:
         nbytes = sum(fields[best]) * C
     achieved = nbytes / avg_s / 1e9
@@ -293,7 +298,7 @@ def roofline_of(res, N, precision, les=True):
             continue
         entry = {"ms_per_launch": round(t / c, 4), "launches": c}
         if k in fields:
-            b = xfwd_mean_fields(les) * C if k == "x_forward_update" else sum(fields[k]) * C
+            b = xfwd_mean_fields(les, fused) * C if k == "x_forward_update" else sum(fields[k]) * C
             entry["algorithmic_GBps"] = round(b / (t / c / 1e3) / 1e9, 1)
         per_kernel[k] = entry
     return {

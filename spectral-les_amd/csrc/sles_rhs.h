@@ -605,9 +605,11 @@ __device__ __forceinline__ double phi2_d(double z) {
   return (expm1(z) - z) / (z * z);
 }
 
+// ynext (RK4 stages 0-2): also receives the next stage's input U_hat.
 template <typename T>
 __device__ __forceinline__ void epilogue(cpx<T> (&s)[3], eidx e, int kx, int ky, int kz, int n, const StepArgs& a,
-                                         double dt, cpx<T>* uh, cpx<T>* uh0, cpx<T>* uh1, cpx<T>* aux, long C) {
+                                         double dt, cpx<T>* uh, cpx<T>* uh0, cpx<T>* uh1, cpx<T>* aux, long C,
+                                         cpx<T> (&ynext)[3]) {
   const KIndex k = {kx, ky, kz};
   const T keep = dealias_keep(k, n) ? T(1) : T(0);
   const long k2 = (long)kx * kx + (long)ky * ky + (long)kz * kz;
@@ -672,11 +674,13 @@ __device__ __forceinline__ void epilogue(cpx<T> (&s)[3], eidx e, int kx, int ky,
         stx(u1i, e, u[i] + ca * d[i]);
         stx(ui, e, y);
         if (a.stages) stx((cpx<T>*)a.stages + i * C, e, y);
+        ynext[i] = y;
       } else if (a.stage < 3) {
         const cpx<T> y = ldx(u0i, e) + cb * d[i];
         stx(u1i, e, ldx(u1i, e) + ca * d[i]);
         stx(ui, e, y);
         if (a.stages) stx((cpx<T>*)a.stages + (3 * a.stage + i) * C, e, y);
+        ynext[i] = y;
       } else {
         const cpx<T> fin = ldx(u1i, e) + ca * d[i];
         stx(u1i, e, fin);
@@ -711,7 +715,15 @@ __device__ __forceinline__ void epilogue(cpx<T> (&s)[3], eidx e, int kx, int ky,
 template <typename T, int N, class PL>
 using XShape = StridedShape<T, N, PL, (sizeof(T) == 4 && PL::P == 4) ? 1024 : 512>;
 
-template <typename T, int N, int LES, class PL>
+// FUSE (RK4 stages 0-2, FP32 form): after the update, the block also runs
+// the next stage's x-inverse on the new stage input it holds (same x-lines,
+// same columns: no other block touches them), writing W as k_xinv would, so
+// the next stage skips k_xinv and the re-read of U_hat.  FP32 256^3: 0.479 ms
+// per fused launch against 0.372 + 0.146 ms separately (step +2 %).  In FP64
+// the fused inverse runs on the x-forward's radix-4 plan at 2 waves per SIMD
+// and was slower than the separate radix-8 k_xinv (1.071 against 1.038 ms),
+// so FP64 keeps the two kernels.
+template <typename T, int N, int LES, class PL, bool FUSE = false>
 __global__ void __launch_bounds__((XShape<T, N, PL>::THREADS))
     k_xfwd(cpx<T>* __restrict__ W, long C, cpx<T>* uh, cpx<T>* uh0, cpx<T>* uh1, cpx<T>* aux, StepArgs a,
            const cpx<T>* __restrict__ tw, int xcd) {
@@ -773,12 +785,52 @@ __global__ void __launch_bounds__((XShape<T, N, PL>::THREADS))
       }
     }
   }
-  if (!col.valid) return;
   const int ky = kfreq((int)(col.g / NH), N), kz = (int)(col.g % NH);
+  cpx<T> y0[P], y1[P], y2[P];
 #pragma unroll
   for (int m = 0; m < P; ++m) {
     cpx<T> s3[3] = {a0[m], a1[m], a2[m]};
-    epilogue<T>(s3, at(m), kfreq(t + m * NT, N), ky, kz, N, a, dt, uh, uh0, uh1, aux, C);
+    cpx<T> yn[3] = {{T(0), T(0)}, {T(0), T(0)}, {T(0), T(0)}};
+    if (col.valid) epilogue<T>(s3, at(m), kfreq(t + m * NT, N), ky, kz, N, a, dt, uh, uh0, uh1, aux, C, yn);
+    y0[m] = yn[0];
+    y1[m] = yn[1];
+    y2[m] = yn[2];
+  }
+  if constexpr (FUSE) {
+    using IFFT = LineFFTP<T, PL, +1, false, PPB>;
+    auto kx = [&](int m) { return (T)kfreq(t + m * NT, N); };
+    auto ifft_store = [&](cpx<T>(&x)[P], int j) {
+      IFFT::run(x, lds, map, t, twr);
+      if (col.valid)
+#pragma unroll
+        for (int m = 0; m < P; ++m) stx(W + j * C, at(m), x[m]);
+    };
+    if constexpr (LES) {
+      // as k_xinv: W_j = F_x^-1 Y_j, W_{3+j} = F_x^-1 (1j Kx Y_j)
+#pragma unroll
+      for (int m = 0; m < P; ++m) v[m] = mul_i(kx(m) * y0[m]);
+      ifft_store(v, 3);
+      ifft_store(y0, 0);
+#pragma unroll
+      for (int m = 0; m < P; ++m) v[m] = mul_i(kx(m) * y1[m]);
+      ifft_store(v, 4);
+      ifft_store(y1, 1);
+#pragma unroll
+      for (int m = 0; m < P; ++m) v[m] = mul_i(kx(m) * y2[m]);
+      ifft_store(v, 5);
+      ifft_store(y2, 2);
+    } else {
+      // as k_xinv without LES: W0-2 = F_x^-1 Y, W3 = F_x^-1 (1j Kx Y_1), W4 = F_x^-1 (1j Kx Y_2)
+#pragma unroll
+      for (int m = 0; m < P; ++m) v[m] = mul_i(kx(m) * y1[m]);
+      ifft_store(v, 3);
+#pragma unroll
+      for (int m = 0; m < P; ++m) v[m] = mul_i(kx(m) * y2[m]);
+      ifft_store(v, 4);
+      ifft_store(y0, 0);
+      ifft_store(y1, 1);
+      ifft_store(y2, 2);
+    }
   }
 }
 
@@ -871,8 +923,12 @@ struct Fused {
   using SS = WideShape<T, N, PS>;  // x/y inverse and y forward
   using SX = XShape<T, N, PX>;  // x forward + stage update
   static constexpr int ZB = (int)(((long)N * N + ZLines<PZ>::LPB - 1) / ZLines<PZ>::LPB);
+  // RK4 x-forward runs the next stage's x-inverse (see k_xfwd): FP32 form only
+  static constexpr bool FUSE_X = sizeof(T) == 4;
 
-  static int stage(const sles_plan_s* p, int les, T coeff, cpx<T>* uh, cpx<T>* W, double* part, hipStream_t s) {
+  // skip_xinv: W0-5 already hold the x-inverse output (fused into the previous x-forward)
+  static int stage(const sles_plan_s* p, int les, T coeff, cpx<T>* uh, cpx<T>* W, double* part, hipStream_t s,
+                   bool skip_xinv = false) {
     const long C = ncplx(p);
     const cpx<T>* tw = (const cpx<T>*)p->tw;
     const int xcd = p->tune_xcd;
@@ -880,7 +936,10 @@ struct Fused {
     const dim3 gs((unsigned)((NCOL + SS::TK - 1) / SS::TK)), bs(SS::THREADS);
     const T scale = T(1.0 / ((double)N * N * N));
     if (les) {
-      { ProfScope ps_(PK_XINV, s); hipLaunchKernelGGL((k_xinv<T, N, 1, PS>), gs, bs, 0, s, uh, W, C, tw, xcd); }
+      if (!skip_xinv) {
+        ProfScope ps_(PK_XINV, s);
+        hipLaunchKernelGGL((k_xinv<T, N, 1, PS>), gs, bs, 0, s, uh, W, C, tw, xcd);
+      }
       { ProfScope ps_(PK_YINV, s); hipLaunchKernelGGL((k_yinv<T, N, 1, PS>), gs, bs, 0, s, W, C, tw, xcd); }
       ProfScope ps_(PK_Z, s);
       if (les == SLES_LES_SMAGORINSKY)
@@ -890,7 +949,10 @@ struct Fused {
         hipLaunchKernelGGL((k_zles<T, N, PZ, SLES_LES_VREMAN>), dim3(ZB), dim3(ZLines<PZ>::THREADS), 0, s, W, C,
                            scale, coeff, part, tw);
     } else {
-      { ProfScope ps_(PK_XINV, s); hipLaunchKernelGGL((k_xinv<T, N, 0, PS>), gs, bs, 0, s, uh, W, C, tw, xcd); }
+      if (!skip_xinv) {
+        ProfScope ps_(PK_XINV, s);
+        hipLaunchKernelGGL((k_xinv<T, N, 0, PS>), gs, bs, 0, s, uh, W, C, tw, xcd);
+      }
       { ProfScope ps_(PK_YINV, s); hipLaunchKernelGGL((k_yinv<T, N, 0, PS>), gs, bs, 0, s, W, C, tw, xcd); }
       ProfScope ps_(PK_Z, s);
       hipLaunchKernelGGL((k_zns<T, N, PZ>), dim3(ZB), dim3(ZLines<PZ>::THREADS), 0, s, W, C, scale, part, tw);
@@ -898,8 +960,9 @@ struct Fused {
     return check_launch("fused stage (inverse)") ? -1 : 0;
   }
 
+  // fuse_next: also run the next stage's x-inverse (RK4 stages 0-2)
   static int finish(const sles_plan_s* p, int les, cpx<T>* uh, cpx<T>* uh0, cpx<T>* uh1, cpx<T>* aux, cpx<T>* W,
-                    const StepArgs& a, hipStream_t s) {
+                    const StepArgs& a, hipStream_t s, bool fuse_next = false) {
     const long C = ncplx(p);
     const cpx<T>* tw = (const cpx<T>*)p->tw;
     const int xcd = p->tune_xcd;
@@ -908,12 +971,22 @@ struct Fused {
     const dim3 gx((unsigned)((NCOL + SX::TK - 1) / SX::TK)), bx(SX::THREADS);
     if (les) {
       { ProfScope ps_(PK_YFWD, s); hipLaunchKernelGGL((k_yfwd<T, N, 1, PS>), gs, bs, 0, s, W, C, tw, xcd); }
-      { ProfScope ps_(PK_XFWD, s);
-        hipLaunchKernelGGL((k_xfwd<T, N, 1, PX>), gx, bx, 0, s, W, C, uh, uh0, uh1, aux, a, tw, xcd); }
+      ProfScope ps_(PK_XFWD, s);
+      if constexpr (FUSE_X)
+        if (fuse_next) {
+          hipLaunchKernelGGL((k_xfwd<T, N, 1, PX, true>), gx, bx, 0, s, W, C, uh, uh0, uh1, aux, a, tw, xcd);
+          return check_launch("fused stage (forward)");
+        }
+      hipLaunchKernelGGL((k_xfwd<T, N, 1, PX>), gx, bx, 0, s, W, C, uh, uh0, uh1, aux, a, tw, xcd);
     } else {
       { ProfScope ps_(PK_YFWD, s); hipLaunchKernelGGL((k_yfwd<T, N, 0, PS>), gs, bs, 0, s, W, C, tw, xcd); }
-      { ProfScope ps_(PK_XFWD, s);
-        hipLaunchKernelGGL((k_xfwd<T, N, 0, PX>), gx, bx, 0, s, W, C, uh, uh0, uh1, aux, a, tw, xcd); }
+      ProfScope ps_(PK_XFWD, s);
+      if constexpr (FUSE_X)
+        if (fuse_next) {
+          hipLaunchKernelGGL((k_xfwd<T, N, 0, PX, true>), gx, bx, 0, s, W, C, uh, uh0, uh1, aux, a, tw, xcd);
+          return check_launch("fused stage (forward)");
+        }
+      hipLaunchKernelGGL((k_xfwd<T, N, 0, PX>), gx, bx, 0, s, W, C, uh, uh0, uh1, aux, a, tw, xcd);
     }
     return check_launch("fused stage (forward)");
   }
@@ -974,7 +1047,9 @@ int step_impl(const sles_plan_s* p, int scheme, const sles_step_params* prm, voi
     a.ca = rk4a[st];
     a.cb = rk4b[st];
     double* pp = (ddt && st == 0) ? part : nullptr;
-    if (F::stage(p, les, coeff, uh, W, pp, s)) return -1;
+    const bool fuse = F::FUSE_X && scheme == SLES_SCHEME_RK4;
+    const bool fused_in = fuse && st > 0;  // x-inverse ran in the previous x-forward
+    if (F::stage(p, les, coeff, uh, W, pp, s, fused_in)) return -1;
     if (pp) {
       // two levels: up to 64 chunks of the per-block maxima, then one block
       // (the partials area holds N^2 + 64 doubles; the z grid is at most N^2)
@@ -987,7 +1062,7 @@ int step_impl(const sles_plan_s* p, int scheme, const sles_step_params* prm, voi
                          scalars);
       if (check_launch("cfl")) return -1;
     }
-    if (F::finish(p, les, uh, uh0, uh1, aux, W, a, s)) return -1;
+    if (F::finish(p, les, uh, uh0, uh1, aux, W, a, s, fuse && st < 3)) return -1;
   }
   if (u_out || curl_out) return F::physical(p, uh, W, u_out, curl_out, s);
   return 0;
