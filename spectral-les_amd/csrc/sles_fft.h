@@ -277,6 +277,18 @@ struct TwTable {
   }
 };
 
+// Table twiddles addressed as a uniform base plus a 32-bit offset (one VGPR
+// per factor address instead of two), for kernels at the register limit.
+template <typename T>
+struct TwTable32 {
+  const cpx<T>* __restrict__ tw;
+  template <int N, int NT, int Ns, int R, int OFF>
+  __device__ __forceinline__ cpx<T> get(int q, int r, int t) const {
+    const int k = (t + q * NT) % Ns;
+    return ld32(tw, (unsigned)(r * k * (N / (Ns * R))));
+  }
+};
+
 // Number of register twiddles of a plan (stages after the first).
 template <int P, int Ns, int... Rs>
 struct TwCount;

@@ -123,22 +123,39 @@ __global__ void __launch_bounds__(ZShape<N>::THREADS)
 // Strided complex pass: batch of fields, each viewed as (outer, N, M) with the
 // transform along the middle axis.  Column g in [0, outer*M) -> (g / M, g % M).
 // One block per tile of TK columns runs every field of the batch, the next
-// field's loads issued before the current field's transform.
+// field's loads issued before the current field's transform.  S: tile shape,
+// PL: line plan, TWR: twiddles in registers; else twiddles from the table and
+// one field per block (grid.y = fields).
 // ---------------------------------------------------------------------------
-template <typename T, int N, int DIR>
-__global__ void __launch_bounds__((WideShape<T, N>::THREADS))
+template <typename T, int N, int DIR, class S = WideShape<T, N>, class PL = LinePlan<N>, bool TWR = true>
+__global__ void __launch_bounds__((S::THREADS))
     k_strided(const cpx<T>* in, cpx<T>* out, long outer, long M, long fstride, int nfields, T scale,
               const cpx<T>* __restrict__ tw, int xcd) {
-  using S = WideShape<T, N>;
-  constexpr int P = S::P, NT = S::NT, TK = S::TK;
+  constexpr int P = PL::P, NT = PL::NT, TK = S::TK;
+  static_assert(S::P == P && S::THREADS == TK * NT, "tile shape and line plan disagree");
   __shared__ cpx<T> lds[N * TK];
   const int c = threadIdx.x % TK, t = threadIdx.x / TK;
-  const TwRegs<T, LinePlan<N>> twr(tw, t);  // issued together with the data loads
+  const auto twr = TwPick<T, PL, TWR>::make(tw, t);  // issued together with the data loads
   const long g = tile_id(xcd) * TK + c;
   const bool valid = g < outer * M;
   const long o = valid ? g / M : 0, col = valid ? g % M : 0;
   const long base = o * (long)N * M + col;
   cpx<T> v[P], nx[P];
+  if constexpr (!TWR) {
+    // Table-twiddle (wide) tiles: one field per block (blockIdx.y), element
+    // offsets within the field in 32 bits (a field is below 4 GiB: N <= 512)
+    // and a uniform field base pointer, to stay inside 128 VGPRs.
+    const unsigned b32 = (unsigned)base, m32 = (unsigned)M;
+    const long fo = (long)blockIdx.y * fstride;
+#pragma unroll
+    for (int m = 0; m < P; ++m) v[m] = ld32(in + fo, b32 + (unsigned)(t + m * NT) * m32);
+    LineFFTP<T, PL, DIR>::S::run(v, lds, LdsColumn<TK>{c}, t, TwTable32<T>{tw});
+    if (valid) {
+#pragma unroll
+      for (int m = 0; m < P; ++m) st32(out + fo, b32 + (unsigned)(t + m * NT) * m32, scale * v[m]);
+    }
+    return;
+  }
 #pragma unroll
   for (int m = 0; m < P; ++m) nx[m] = in[base + (long)(t + m * NT) * M];  // in bounds (clamped); unused if !valid
 #pragma unroll 1
@@ -149,13 +166,41 @@ __global__ void __launch_bounds__((WideShape<T, N>::THREADS))
 #pragma unroll
       for (int m = 0; m < P; ++m) nx[m] = in[(f + 1) * fstride + base + (long)(t + m * NT) * M];
     }
-    LineFFT<T, N, DIR>::run(v, lds, LdsColumn<TK>{c}, t, twr);
+    LineFFTP<T, PL, DIR>::run(v, lds, LdsColumn<TK>{c}, t, twr);
     if (valid) {
 #pragma unroll
       for (int m = 0; m < P; ++m) out[f * fstride + base + (long)(t + m * NT) * M] = scale * v[m];
     }
   }
 }
+
+// Wide tiles for N = 256, 512: twice the columns of WideShape (LDS tile up to
+// 128 KiB, 1024 threads), so every row of a tile is a run of 256 B; twiddles
+// come from the table to stay inside the 128 VGPRs of a 1024-thread block.
+// FP32 at N = 512 holds 16 points per thread (radix-8 stages, two butterflies
+// per thread per stage) to keep 32 columns within 1024 threads.
+template <typename T, int N>
+struct WidePlan {
+  using type = LinePlan<N>;
+};
+template <>
+struct WidePlan<float, 512> {
+  using type = Plan<512, 16, 8, 8, 8>;
+};
+template <typename T, int N>
+struct WideTile {
+  using PL = typename WidePlan<T, N>::type;
+  static constexpr int P = PL::P;
+  static constexpr int NT = PL::NT;
+  static constexpr int TKL = 131072 / (N * (int)sizeof(cpx<T>));
+  static constexpr int TKT = 1024 / NT;
+  static constexpr int TK = TKL < TKT ? TKL : TKT;
+  static constexpr int THREADS = TK * NT;
+};
+template <int N>
+struct HasWide {
+  static constexpr bool value = N == 256 || N == 512;
+};
 
 // ---------------------------------------------------------------------------
 // Host launchers
@@ -182,17 +227,35 @@ static int launch_z_c2r(const sles_plan_s* p, const void* in, void* out, long nr
   return check_launch("z_c2r");
 }
 
+template <typename T, int N, class S, class PL, bool TWR>
+static void launch_strided_shape(const sles_plan_s* p, const void* in, void* out, long outer, long M,
+                                 long fstride, int nfields, int dir, double scale, hipStream_t s) {
+  const long blocks = (outer * M + S::TK - 1) / S::TK;
+  // table-twiddle tiles run one field per block row (grid.y)
+  const dim3 grid((unsigned)blocks, TWR ? 1u : (unsigned)nfields);
+  if (dir < 0)
+    hipLaunchKernelGGL((k_strided<T, N, -1, S, PL, TWR>), grid, dim3(S::THREADS), 0, s,
+                       (const cpx<T>*)in, (cpx<T>*)out, outer, M, fstride, nfields, (T)scale,
+                       (const cpx<T>*)p->tw, p->tune_xcd);
+  else
+    hipLaunchKernelGGL((k_strided<T, N, +1, S, PL, TWR>), grid, dim3(S::THREADS), 0, s,
+                       (const cpx<T>*)in, (cpx<T>*)out, outer, M, fstride, nfields, (T)scale,
+                       (const cpx<T>*)p->tw, p->tune_xcd);
+}
+
+// wide: use WideTile (sizes that have it)
 template <typename T, int N>
 static int launch_strided(const sles_plan_s* p, const void* in, void* out, long outer, long M,
-                          long fstride, int nfields, int dir, double scale, hipStream_t s) {
-  using S = WideShape<T, N>;
-  const long blocks = (outer * M + S::TK - 1) / S::TK;
-  if (dir < 0)
-    hipLaunchKernelGGL((k_strided<T, N, -1>), dim3((unsigned)blocks), dim3(S::THREADS), 0, s, (const cpx<T>*)in,
-                       (cpx<T>*)out, outer, M, fstride, nfields, (T)scale, (const cpx<T>*)p->tw, p->tune_xcd);
-  else
-    hipLaunchKernelGGL((k_strided<T, N, +1>), dim3((unsigned)blocks), dim3(S::THREADS), 0, s, (const cpx<T>*)in,
-                       (cpx<T>*)out, outer, M, fstride, nfields, (T)scale, (const cpx<T>*)p->tw, p->tune_xcd);
+                          long fstride, int nfields, int dir, double scale, bool wide, hipStream_t s) {
+  if constexpr (HasWide<N>::value) {
+    if (wide && nfields <= 65535) {
+      using W = WideTile<T, N>;
+      launch_strided_shape<T, N, W, typename W::PL, false>(p, in, out, outer, M, fstride, nfields, dir, scale, s);
+      return check_launch("strided (wide)");
+    }
+  }
+  launch_strided_shape<T, N, WideShape<T, N>, LinePlan<N>, true>(p, in, out, outer, M, fstride, nfields, dir,
+                                                                 scale, s);
   return check_launch("strided");
 }
 
@@ -202,12 +265,13 @@ static int fft_r2c_t(const sles_plan_s* p, const void* in, void* out, int batch,
   const int n = p->n;
   const long C = ncplx(p);
   const long rows = (long)batch * n * n;
+  const bool wx = (p->tune_fft_wide & 1) != 0, wy = (p->tune_fft_wide & 2) != 0;
   int rc = 0;
 #define SLES_CASE(NN)                                                                     \
   case NN:                                                                                \
     rc = launch_z_r2c<T, NN>(p, in, out, rows, 1.0, s);                                   \
-    if (!rc) rc = launch_strided<T, NN>(p, out, out, (long)n, (long)p->nh, C, batch, -1, 1.0, s); \
-    if (!rc) rc = launch_strided<T, NN>(p, out, out, 1, (long)n * p->nh, C, batch, -1, scale, s); \
+    if (!rc) rc = launch_strided<T, NN>(p, out, out, (long)n, (long)p->nh, C, batch, -1, 1.0, wy, s); \
+    if (!rc) rc = launch_strided<T, NN>(p, out, out, 1, (long)n * p->nh, C, batch, -1, scale, wx, s); \
     break;
   switch (n) {
     SLES_FOR_EACH_N(SLES_CASE)
@@ -224,11 +288,12 @@ static int fft_c2r_t(const sles_plan_s* p, const void* in, void* out, void* work
   const int n = p->n;
   const long C = ncplx(p);
   const long rows = (long)batch * n * n;
+  const bool wx = (p->tune_fft_wide & 1) != 0, wy = (p->tune_fft_wide & 2) != 0;
   int rc = 0;
 #define SLES_CASE(NN)                                                                      \
   case NN:                                                                                 \
-    rc = launch_strided<T, NN>(p, in, work, 1, (long)n * p->nh, C, batch, +1, 1.0, s);      \
-    if (!rc) rc = launch_strided<T, NN>(p, work, work, (long)n, (long)p->nh, C, batch, +1, 1.0, s); \
+    rc = launch_strided<T, NN>(p, in, work, 1, (long)n * p->nh, C, batch, +1, 1.0, wx, s);  \
+    if (!rc) rc = launch_strided<T, NN>(p, work, work, (long)n, (long)p->nh, C, batch, +1, 1.0, wy, s); \
     if (!rc) rc = launch_z_c2r<T, NN>(p, work, out, rows, scale, s);                       \
     break;
   switch (n) {
@@ -280,6 +345,10 @@ int sles_plan_create(sles_plan* out, int n, int precision) {
   p->n = n;
   p->nh = n / 2 + 1;
   p->precision = precision;
+  // wide tiles of the generic transforms where they measured faster (MI355X,
+  // batch of 3 fields): both passes at 512, the x pass (FP64) or the y pass
+  // (FP32) at 256
+  p->tune_fft_wide = n == 512 ? 3 : n == 256 ? (precision == SLES_FP64 ? 1 : 2) : 0;
   (void)hipGetDevice(&p->device);
   // Twiddles exp(-2 pi i e/n) evaluated in long double and rounded once.
   const size_t esz = precision == SLES_FP64 ? 16 : 8;
@@ -330,6 +399,10 @@ int sles_plan_set_tuning(sles_plan p, int key, int value) {
     case SLES_TUNE_XFWD_RADIX4: p->tune_xfwd = value != 0; return 0;
     case SLES_TUNE_Z_RADIX4: p->tune_z = value != 0; return 0;
     case SLES_TUNE_XCD: p->tune_xcd = value != 0; return 0;
+    case SLES_TUNE_FFT_WIDE:
+      if (value < 0 || value > 3) return fail("sles_plan_set_tuning: SLES_TUNE_FFT_WIDE takes 0..3");
+      p->tune_fft_wide = value;
+      return 0;
     default: return fail("sles_plan_set_tuning: unknown key");
   }
 }
@@ -341,6 +414,7 @@ int sles_plan_get_tuning(sles_plan p, int key, int* value) {
     case SLES_TUNE_XFWD_RADIX4: *value = p->tune_xfwd; return 0;
     case SLES_TUNE_Z_RADIX4: *value = p->tune_z; return 0;
     case SLES_TUNE_XCD: *value = p->tune_xcd; return 0;
+    case SLES_TUNE_FFT_WIDE: *value = p->tune_fft_wide; return 0;
     default: return fail("sles_plan_get_tuning: unknown key");
   }
 }

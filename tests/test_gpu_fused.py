@@ -168,6 +168,6 @@ def test_every_tuning_variant_matches_oracle(les, precision):
         g.U_hat.copy_(start)
         g.U_hat1.copy_(start)
         g.plan.set_tuning(**st)
-        assert g.plan.get_tuning() == st
+        assert {k: v for k, v in g.plan.get_tuning().items() if k in st} == st
         fused.rk4_step(g)
         assert rel(g.U_hat, o.U_hat) < tol, st

@@ -45,6 +45,43 @@ def test_fft_forward_backward_vs_numpy(N, precision):
     assert rel(x2, refx[0] * N ** 3) < tol
 
 
+@pytest.mark.parametrize(
+    "N,precision,wide",
+    [(256, "fp64", 3), (256, "fp32", 3), (256, "fp64", 1), (256, "fp32", 2), (512, "fp64", 3), (512, "fp32", 3)],
+)
+def test_fft_wide_tiles_vs_numpy(N, precision, wide):
+    """Wide-tile strided passes (SLES_TUNE_FFT_WIDE; bit 0 x pass, bit 1 y
+    pass) against numpy and against the default tiles on the same input."""
+    from sles.initialize_domain import InitializeDomain
+
+    d = InitializeDomain(N=N, precision=precision, method="RK4")
+    B = 2 if N <= 256 else 1
+    rng = np.random.default_rng(N + wide)
+    u = rng.standard_normal((B, N, N, N))
+    Z = (rng.standard_normal((B, N, N, N // 2 + 1)) + 1j * rng.standard_normal((B, N, N, N // 2 + 1))) * 1e-3
+    ut, Zt = torch.as_tensor(u, device="cuda"), torch.as_tensor(Z, device="cuda")
+    tol = 1e-13 if precision == "fp64" else 3e-6
+    default = d.plan.get_tuning()["fft_wide"]
+    assert default == (3 if N == 512 else 1 if precision == "fp64" else 2)
+    d.plan.set_tuning(fft_wide=0)
+    X0 = d.fft.forward(ut).clone()
+    x0 = d.fft.backward(Zt).clone()
+    d.plan.set_tuning(fft_wide=wide)
+    try:
+        assert d.plan.get_tuning()["fft_wide"] == wide
+        X = d.fft.forward(ut)
+        ref = np.fft.rfftn(u, axes=(1, 2, 3))
+        assert rel(X, ref) < tol
+        assert rel(X, X0) < tol
+        del ref
+        x = d.fft.backward(Zt)
+        refx = np.fft.irfftn(Z, s=(N, N, N), axes=(1, 2, 3))
+        assert rel(x, refx) < tol
+        assert rel(x, x0) < tol
+    finally:
+        d.plan.set_tuning(fft_wide=default)
+
+
 @pytest.mark.parametrize("N", [16, 256])
 def test_fft_large_roundtrip(N):
     from sles.initialize_domain import InitializeDomain
