@@ -7,6 +7,7 @@ ensemble         many members of different grid sizes advanced together, one
                  captured HIP graph per ensemble step (see ensemble.py).
 ensemble-adjoint d(loss)/dCs of every member through the hand-written adjoint.
 differentiable   Vreman ETD2 gradients against finite differences.
+fft              generic 3D transforms (batch of 3 fields): ms and GB/s per call.
 
     python -m sles.studies precision --n 256 --steps 100 500 --out result.json
 """
@@ -324,6 +325,51 @@ def ensemble_adjoint(groups=((64, 8), (96, 8), (128, 8)), steps=200, checkpoint_
     return out
 
 
+def fft_throughput(sizes=(256, 512), precisions=("fp64", "fp32"), batch=3, reps=10):
+    """Generic 3D transforms (sles_fft_r2c / sles_fft_c2r) of `batch` fields
+    with each plan's default tiles: ms per call and algorithmic GB/s,
+    batch (R + 5 C) bytes per call (z pass R in / C out, y and x passes C in /
+    C out, each field once)."""
+    from . import _lib
+
+    out = {}
+    for N in sizes:
+        for pr in precisions:
+            fp64 = pr == "fp64"
+            plan = _lib.Plan(N, _lib.SLES_FP64 if fp64 else _lib.SLES_FP32, torch.device("cuda"))
+            s = 8 if fp64 else 4
+            R, C = N ** 3 * s, N * N * (N // 2 + 1) * 2 * s
+            u = torch.randn(batch, N, N, N, dtype=torch.float64 if fp64 else torch.float32, device="cuda")
+            U = torch.empty(batch, N, N, N // 2 + 1, dtype=torch.complex128 if fp64 else torch.complex64,
+                            device="cuda")
+            st = torch.cuda.current_stream().cuda_stream
+            calls = {
+                "r2c": lambda: _lib.call("sles_fft_r2c", plan.handle, u.data_ptr(), U.data_ptr(), batch, 1.0, st),
+                "c2r": lambda: _lib.call("sles_fft_c2r", plan.handle, U.data_ptr(), u.data_ptr(), U.data_ptr(),
+                                         batch, 1.0 / N ** 3, st),
+            }
+            res = {"tuning": plan.get_tuning()}
+            for name, fn in calls.items():
+                fn()
+                torch.cuda.synchronize()
+                e0 = torch.cuda.Event(enable_timing=True)
+                e1 = torch.cuda.Event(enable_timing=True)
+                e0.record()
+                for _ in range(reps):
+                    fn()
+                e1.record()
+                e1.synchronize()
+                ms = e0.elapsed_time(e1) / reps
+                gbs = batch * (R + 5 * C) / (ms * 1e-3) / 1e9
+                res[name] = {"ms": round(ms, 4), "algorithmic_GBps": round(gbs, 1),
+                             "frac_of_8TBps": round(gbs / 8000.0, 4)}
+            out[f"{N}_{pr}"] = res
+            print(f"{N}_{pr}", json.dumps(res), flush=True)
+            del u, U, plan
+            torch.cuda.empty_cache()
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -351,6 +397,9 @@ def main(argv=None):
     rc.add_argument("--precision", default="fp32")
     rc.add_argument("--steps", type=int, default=20)
     rc.add_argument("--recompute-all", action="store_true", help="recompute the last segment too")
+    ff = sub.add_parser("fft")
+    ff.add_argument("--n", type=int, nargs="+", default=[256, 512])
+    ff.add_argument("--reps", type=int, default=10)
     ra = sub.add_parser("ensemble-adjoint")
     ra.add_argument("--steps", type=int, default=200)
     ra.add_argument("--out", default=None)
@@ -360,6 +409,9 @@ def main(argv=None):
         if a.out:
             with open(a.out, "w") as f:
                 json.dump(res, f, indent=1)
+        return
+    if a.cmd == "fft":
+        fft_throughput(tuple(a.n), reps=a.reps)
         return
     if a.cmd == "precision-tgv":
         precision_tgv(a.n, a.steps)
