@@ -93,24 +93,43 @@ __global__ void __launch_bounds__(ZShape<N>::THREADS)
   const long ra = 2 * ((long)blockIdx.x * S::LPB + line), rb = ra + 1;
   const bool va = ra < nrows, vb = rb < nrows;
   const long rla = va ? ra : 0, rlb = vb ? rb : 0;  // clamped rows (in bounds)
-  cpx<T> v[P];
+  constexpr bool WV = WaveLocal<LinePlan<N>>::value;
+  // Each half-spectrum element is read from memory once (bins k <= N/2: slots
+  // m < P/2, plus k = N/2 in thread 0); the Hermitian completion k > N/2 takes
+  // conj of bin N - k from the line's LDS row (unpermuted layout).
+  cpx<T> ha[P / 2 + 1], hb[P / 2 + 1];
 #pragma unroll
-  for (int m = 0; m < P; ++m) {
+  for (int m = 0; m <= P / 2; ++m) {
     const int k = t + m * NT;
-    const int kk = k <= N / 2 ? k : N - k;
-    cpx<T> a = in[rla * NH + kk], b = in[rlb * NH + kk];
-    if (!va) a = {T(0), T(0)};
-    if (!vb) b = {T(0), T(0)};
-    if (k > N / 2) {
-      a = conj(a);
-      b = conj(b);
-    } else if (k == 0 || k == N / 2) {  // c2r ignores the imaginary part of these bins
-      a.y = T(0);
-      b.y = T(0);
+    if (k <= N / 2) {
+      ha[m] = in[rla * NH + k];
+      hb[m] = in[rlb * NH + k];
+      if (!va) ha[m] = {T(0), T(0)};
+      if (!vb) hb[m] = {T(0), T(0)};
     }
-    v[m] = {a.x - b.y, a.y + b.x};  // a + i b
   }
-  LineFFTP<T, LinePlan<N>, +1, WaveLocal<LinePlan<N>>::value>::run(v, lds, map, t, twr);
+  const LdsPlain pmap = map.plain();
+  cpx<T> v[P];
+  auto complete = [&](const cpx<T>(&h)[P / 2 + 1], bool second) {
+#pragma unroll
+    for (int m = 0; m <= P / 2; ++m)
+      if (t + m * NT <= N / 2) lds[pmap(t + m * NT)] = h[m];
+    line_sync<WV>();
+#pragma unroll
+    for (int m = 0; m < P; ++m) {
+      const int k = t + m * NT;
+      cpx<T> x = lds[pmap(k <= N / 2 ? k : N - k)];
+      if (k > N / 2)
+        x = conj(x);
+      else if (k == 0 || k == N / 2)  // c2r ignores the imaginary part of these bins
+        x.y = T(0);
+      v[m] = second ? cpx<T>{v[m].x - x.y, v[m].y + x.x} : x;  // a + i b
+    }
+    line_sync<WV>();
+  };
+  complete(ha, false);
+  complete(hb, true);
+  LineFFTP<T, LinePlan<N>, +1, WV>::run(v, lds, map, t, twr);
 #pragma unroll
   for (int m = 0; m < P; ++m) {
     const int z = t + m * NT;
