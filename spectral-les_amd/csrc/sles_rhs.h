@@ -246,43 +246,55 @@ template <typename T, int N, class PL>
 constexpr int ZPPB = WaveLocal<PL>::value ? 0 : ZLines<PL>::LPB * LdsLine<T, N>::STRIDE;
 
 // Line pairs Z_k = A_k + i B_k of two half-spectra A, B (uniform field bases,
-// line starting at element row0): pair_issue loads bins kk = min(k, N-k),
-// pair_finish multiplies by i*kz where asked, completes the Hermitian half
-// and drops the imaginary parts of the kz = 0 and N/2 bins (as c2r does).
+// line starting at element row0): pair_issue loads bins k <= N/2 once,
+// pair_finish completes the Hermitian half through LDS, multiplies by i*kz
+// where asked and drops the imaginary parts of the kz = 0 and N/2 bins (as
+// c2r does).
 // Split in two so that the next pair's loads are issued before the current
 // pair's transform.
 template <typename T, int P>
 struct PairRaw {
-  cpx<T> a[P], b[P];
+  cpx<T> a[P / 2 + 1], b[P / 2 + 1];  // bins k = t + m*NT <= N/2 (m = P/2: thread 0 only)
 };
 template <typename T, int N, int P, int NT>
 __device__ __forceinline__ void pair_issue(PairRaw<T, P>& r, const cpx<T>* A, const cpx<T>* B, unsigned row0, int t) {
 #pragma unroll
-  for (int m = 0; m < P; ++m) {
+  for (int m = 0; m <= P / 2; ++m) {
     const int k = t + m * NT;
-    const int kk = k <= N / 2 ? k : N - k;
-    r.a[m] = ld32(A, row0 + kk);
-    r.b[m] = ld32(B, row0 + kk);
+    if (k <= N / 2) {
+      r.a[m] = ld32(A, row0 + k);
+      r.b[m] = ld32(B, row0 + k);
+    }
   }
 }
-template <typename T, int N, int P, int NT>
-__device__ __forceinline__ void pair_finish(cpx<T> (&v)[P], const PairRaw<T, P>& r, bool dzA, bool dzB, int t) {
+// Each half-spectrum element is loaded once; the completion k > N/2 takes
+// conj of bin N - k from the line's LDS row (pmap: the unpermuted layout).
+// The row must be free on entry (the previous transform ended with its
+// barrier); it is free again on return.
+template <typename T, int N, int P, int NT, bool WAVE, class Map>
+__device__ __forceinline__ void pair_finish(cpx<T> (&v)[P], const PairRaw<T, P>& r, bool dzA, bool dzB, int t,
+                                            cpx<T>* lds, const Map& pmap) {
+  auto complete = [&](const cpx<T>(&h)[P / 2 + 1], bool dz, bool second) {
 #pragma unroll
-  for (int m = 0; m < P; ++m) {
-    const int k = t + m * NT;
-    const int kk = k <= N / 2 ? k : N - k;
-    cpx<T> a = r.a[m], b = r.b[m];
-    if (dzA) a = mul_i((T)kk * a);
-    if (dzB) b = mul_i((T)kk * b);
-    if (k > N / 2) {
-      a = conj(a);
-      b = conj(b);
-    } else if (k == 0 || k == N / 2) {
-      a.y = T(0);
-      b.y = T(0);
+    for (int m = 0; m <= P / 2; ++m)
+      if (t + m * NT <= N / 2) lds[pmap(t + m * NT)] = h[m];
+    line_sync<WAVE>();
+#pragma unroll
+    for (int m = 0; m < P; ++m) {
+      const int k = t + m * NT;
+      const int kk = k <= N / 2 ? k : N - k;
+      cpx<T> x = lds[pmap(kk)];
+      if (dz) x = mul_i((T)kk * x);
+      if (k > N / 2)
+        x = conj(x);
+      else if (k == 0 || k == N / 2)
+        x.y = T(0);
+      v[m] = second ? cpx<T>{v[m].x - x.y, v[m].y + x.x} : x;  // a + i b
     }
-    v[m] = {a.x - b.y, a.y + b.x};
-  }
+    line_sync<WAVE>();
+  };
+  complete(r.a, dzA, false);
+  complete(r.b, dzB, true);
 }
 
 // After a forward FFT of a + i b held in v (natural layout), split into the
@@ -355,7 +367,7 @@ __global__ void __launch_bounds__(ZLines<PL>::THREADS)
   PairRaw<T, P> pr;
   pair_issue<T, N, P, NT>(pr, w[0], w[1], row0, t);
 #define SLES_INV(DA, DB, NA, NB, RA, RB)                           \
-  pair_finish<T, N, P, NT>(v, pr, DA, DB, t);                      \
+  pair_finish<T, N, P, NT, WV>(v, pr, DA, DB, t, lds, map.plain()); \
   if (NA) pair_issue<T, N, P, NT>(pr, NA, NB, row0, t);            \
   IFFT::run(v, lds, map, t, twr);                                  \
   _Pragma("unroll") for (int m = 0; m < P; ++m) {                  \
@@ -481,7 +493,7 @@ __global__ void __launch_bounds__(ZLines<PL>::THREADS)
   PairRaw<T, P> pr;
   pair_issue<T, N, P, NT>(pr, w[0], w[1], row0, t);
 #define SLES_INV(NA, NB, RA, RB)                                   \
-  pair_finish<T, N, P, NT>(v, pr, false, false, t);                \
+  pair_finish<T, N, P, NT, WV>(v, pr, false, false, t, lds, map.plain()); \
   if (NA) pair_issue<T, N, P, NT>(pr, NA, NB, row0, t);            \
   IFFT::run(v, lds, map, t, twr);                                  \
   _Pragma("unroll") for (int m = 0; m < P; ++m) {                  \
