@@ -300,55 +300,68 @@ __global__ void __launch_bounds__((StridedShape<T, N, PL>::THREADS))
 // ---------------------------------------------------------------------------
 // 3. z kernel
 // ---------------------------------------------------------------------------
-// A pair of half-spectra rows A, B (per-thread row pointers) as Z_k = A_k + i B_k,
-// optionally multiplied by i*kz first, Hermitian completion as c2r.
-template <typename T, int N, int P, int NT>
-__device__ __forceinline__ void load_pair_rows(cpx<T> (&v)[P], const cpx<T>* A, bool dzA, const cpx<T>* B, bool dzB,
-                                               int t, bool valid) {
+// Z_k = a_k + i b_k of two half-spectrum operands, Hermitian completion as
+// c2r: fa(k), fb(k) give the operands at bins k <= N/2 (each evaluated once,
+// from memory); bins k > N/2 take conj of bin N - k from the line's LDS row
+// (pmap: unpermuted layout; free on entry and on return).  HAS_B false: b = 0.
+template <typename T, int N, int P, int NT, bool WAVE, class Map, class FA, class FB>
+__device__ __forceinline__ void complete_pair(cpx<T> (&v)[P], const FA& fa, const FB& fb, bool has_b, int t,
+                                              cpx<T>* lds, const Map& pmap) {
+  // both operands' loads are issued before the first LDS round
+  cpx<T> ha[P / 2 + 1], hb[P / 2 + 1];
 #pragma unroll
-  for (int m = 0; m < P; ++m) {
+  for (int m = 0; m <= P / 2; ++m) {
     const int k = t + m * NT;
-    // A, B point at an in-bounds row even for an invalid line (never stored)
-    const int kk = k <= N / 2 ? k : N - k;
-    cpx<T> a = A[kk], b = B[kk];
-    if (dzA) a = mul_i((T)kk * a);
-    if (dzB) b = mul_i((T)kk * b);
-    if (k > N / 2) {
-      a = conj(a);
-      b = conj(b);
-    } else if (k == 0 || k == N / 2) {
-      a.y = T(0);
-      b.y = T(0);
+    if (k <= N / 2) {
+      ha[m] = fa(k);
+      if (has_b) hb[m] = fb(k);
     }
-    v[m] = {a.x - b.y, a.y + b.x};
   }
+  auto round = [&](const cpx<T>(&h)[P / 2 + 1], bool second) {
+#pragma unroll
+    for (int m = 0; m <= P / 2; ++m)
+      if (t + m * NT <= N / 2) lds[pmap(t + m * NT)] = h[m];
+    line_sync<WAVE>();
+#pragma unroll
+    for (int m = 0; m < P; ++m) {
+      const int k = t + m * NT;
+      cpx<T> x = lds[pmap(k <= N / 2 ? k : N - k)];
+      if (k > N / 2)
+        x = conj(x);
+      else if (k == 0 || k == N / 2)
+        x.y = T(0);
+      v[m] = second ? cpx<T>{v[m].x - x.y, v[m].y + x.x} : x;
+    }
+    line_sync<WAVE>();
+  };
+  round(ha, false);
+  if (has_b) round(hb, true);
 }
 
-// One c2r operand: (F1 [+ (-2i kz) F2]) / Wz(k), Hermitian completion as c2r.
+// A pair of half-spectra rows A, B (row pointers in bounds even for an
+// invalid line, never stored), optionally multiplied by i*kz first.
+template <typename T, int N, int P, int NT, bool WAVE, class Map>
+__device__ __forceinline__ void load_pair_rows(cpx<T> (&v)[P], const cpx<T>* A, bool dzA, const cpx<T>* B, bool dzB,
+                                               int t, cpx<T>* lds, const Map& pmap) {
+  auto fa = [&](int k) { return dzA ? mul_i((T)k * A[k]) : A[k]; };
+  auto fb = [&](int k) { return dzB ? mul_i((T)k * B[k]) : B[k]; };
+  complete_pair<T, N, P, NT, WAVE>(v, fa, fb, true, t, lds, pmap);
+}
+
+// One c2r operand at bin k <= N/2: (F1 [+ (-2i kz) F2]) / Wz(k).
 template <typename T, int N>
 __device__ __forceinline__ cpx<T> adj_term(const cpx<T>* F1, const cpx<T>* F2, int k) {
-  const int kk = k <= N / 2 ? k : N - k;
-  cpx<T> a = F1 ? F1[kk] : cpx<T>{T(0), T(0)};
-  if (F2) a = a + mul_mis(T(2) * (T)kk, F2[kk]);
-  const bool edge = (kk == 0 || kk == N / 2);
-  if (!edge) a = T(0.5) * a;
-  if (k > N / 2)
-    a = conj(a);
-  else if (edge)
-    a.y = T(0);
-  return a;
+  cpx<T> a = F1 ? F1[k] : cpx<T>{T(0), T(0)};
+  if (F2) a = a + mul_mis(T(2) * (T)k, F2[k]);
+  return (k == 0 || k == N / 2) ? a : T(0.5) * a;
 }
 
-template <typename T, int N, int P, int NT>
+template <typename T, int N, int P, int NT, bool WAVE, class Map>
 __device__ __forceinline__ void load_adj_pair(cpx<T> (&v)[P], const cpx<T>* A1, const cpx<T>* A2, const cpx<T>* B1,
-                                              const cpx<T>* B2, int t) {
-#pragma unroll
-  for (int m = 0; m < P; ++m) {
-    const int k = t + m * NT;
-    const cpx<T> a = adj_term<T, N>(A1, A2, k);
-    const cpx<T> b = (B1 || B2) ? adj_term<T, N>(B1, B2, k) : cpx<T>{T(0), T(0)};
-    v[m] = {a.x - b.y, a.y + b.x};
-  }
+                                              const cpx<T>* B2, int t, cpx<T>* lds, const Map& pmap) {
+  auto fa = [&](int k) { return adj_term<T, N>(A1, A2, k); };
+  auto fb = [&](int k) { return adj_term<T, N>(B1, B2, k); };
+  complete_pair<T, N, P, NT, WAVE>(v, fa, fb, B1 || B2, t, lds, pmap);
 }
 
 __device__ __forceinline__ void block_sum_store(double v, double* out) {
@@ -387,7 +400,7 @@ __global__ void __launch_bounds__(ZLines<PL>::THREADS)
   T u0[P], u1[P], u2[P], a[3][3][P];
   cpx<T> v[P];
 #define SLES_INV(PA, DA, PB, DB, RA, RB)                           \
-  load_pair_rows<T, N, P, NT>(v, PA, DA, PB, DB, t, valid);        \
+  load_pair_rows<T, N, P, NT, WV>(v, PA, DA, PB, DB, t, lds, map.plain()); \
   IFFT::run(v, lds, map, t, twr);                                   \
   _Pragma("unroll") for (int m = 0; m < P; ++m) {                  \
     RA[m] = scale * v[m].x;                                        \
@@ -403,7 +416,7 @@ __global__ void __launch_bounds__(ZLines<PL>::THREADS)
   // adjoint sources (unscaled c2r of (.)/Wz)
   T nb0[P], nb1[P], nb2[P], t11[P], t12[P], t22[P], t13[P], t23[P], t33[P];
 #define SLES_AINV(A1, A2, B1, B2, RA, RB)                          \
-  load_adj_pair<T, N, P, NT>(v, A1, A2, B1, B2, t);                \
+  load_adj_pair<T, N, P, NT, WV>(v, A1, A2, B1, B2, t, lds, map.plain()); \
   IFFT::run(v, lds, map, t, twr);                                   \
   _Pragma("unroll") for (int m = 0; m < P; ++m) {                  \
     RA[m] = v[m].x;                                                \
