@@ -27,8 +27,11 @@ def _domain(N, precision, fused, les="Smagorinsky"):
     return d
 
 
-@pytest.mark.parametrize("N", [128, 256])
-@pytest.mark.parametrize("les", ["Smagorinsky", "Vreman"])
+@pytest.mark.parametrize(
+    "N,les",
+    [(128, "Smagorinsky"), (128, "Vreman"), (256, "Smagorinsky"), (256, "Vreman"),
+     (192, "Smagorinsky"), (384, "Vreman")],  # 192, 384: 12-point line plans (radices 4, 4, 4, 3 [, 2])
+)
 def test_fused_vs_operator_path_fp64(N, les):
     import sles.RK4_method as R
 

@@ -45,6 +45,23 @@ def test_fft_forward_backward_vs_numpy(N, precision):
     assert rel(x2, refx[0] * N ** 3) < tol
 
 
+@pytest.mark.parametrize("N", [192, 384])
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_fft_12_point_plans_vs_numpy(N, precision):
+    """Grid sizes with 12 points per thread (radix 4, 4, 4, 3 [, 2] lines)."""
+    from sles.initialize_domain import InitializeDomain
+
+    d = InitializeDomain(N=N, precision=precision, method="RK4")
+    rng = np.random.default_rng(N)
+    u = rng.standard_normal((N, N, N))
+    tol = 1e-13 if precision == "fp64" else 3e-6
+    X = d.fft.forward(torch.as_tensor(u, device="cuda"))
+    assert rel(X, np.fft.rfftn(u)) < tol
+    Z = rng.standard_normal((N, N, N // 2 + 1)) + 1j * rng.standard_normal((N, N, N // 2 + 1))
+    x = d.fft.backward(torch.as_tensor(Z, device="cuda"))
+    assert rel(x, np.fft.irfftn(Z, s=(N, N, N))) < tol
+
+
 @pytest.mark.parametrize(
     "N,precision,wide",
     [(256, "fp64", 3), (256, "fp32", 3), (256, "fp64", 1), (256, "fp32", 2), (512, "fp64", 3), (512, "fp32", 3)],
