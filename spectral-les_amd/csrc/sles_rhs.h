@@ -734,7 +734,7 @@ using XShape = StridedShape<T, N, PL, (sizeof(T) == 4 && PL::P == 4) ? 1024 : 51
 // per fused launch against 0.372 + 0.146 ms separately (step +2 %).  In FP64
 // the fused inverse runs on the x-forward's radix-4 plan at 2 waves per SIMD
 // and was slower than the separate radix-8 k_xinv (1.071 against 1.038 ms),
-// so FP64 keeps the two kernels.
+// so FP64 keeps the two kernels except on launch-bound grids (Fused::FUSE_X).
 template <typename T, int N, int LES, class PL, bool FUSE = false>
 __global__ void __launch_bounds__((XShape<T, N, PL>::THREADS))
     k_xfwd(cpx<T>* __restrict__ W, long C, cpx<T>* uh, cpx<T>* uh0, cpx<T>* uh1, cpx<T>* aux, StepArgs a,
@@ -935,8 +935,11 @@ struct Fused {
   using SS = WideShape<T, N, PS>;  // x/y inverse and y forward
   using SX = XShape<T, N, PX>;  // x forward + stage update
   static constexpr int ZB = (int)(((long)N * N + ZLines<PZ>::LPB - 1) / ZLines<PZ>::LPB);
-  // RK4 x-forward runs the next stage's x-inverse (see k_xfwd): FP32 form only
-  static constexpr bool FUSE_X = sizeof(T) == 4;
+  // RK4 x-forward runs the next stage's x-inverse (see k_xfwd): FP32 form, and
+  // FP64 grids up to 64^3 (launch-bound: one launch fewer per stage; 64^3
+  // Taylor-Green 0.323 -> 0.312 ms per step; at 128^3 FP64 the fused form
+  // was 2 % slower)
+  static constexpr bool FUSE_X = sizeof(T) == 4 || N <= 64;
 
   // skip_xinv: W0-5 already hold the x-inverse output (fused into the previous x-forward)
   static int stage(const sles_plan_s* p, int les, T coeff, cpx<T>* uh, cpx<T>* W, double* part, hipStream_t s,
