@@ -44,8 +44,8 @@ int sles_plan_info(sles_plan plan, int* n, int* nh, int* precision);
 /* Kernel variants of the fused step, chosen per plan (by the autotuner,
  * sles/autotune.py, which persists its choices in a JSON table) and of the
  * generic transforms.  Defaults: 0 for the three plan choices, 1 for
- * SLES_TUNE_XCD; SLES_TUNE_FFT_WIDE 3 at N = 512, 1 (FP64) / 2 (FP32) at
- * N = 256, 0 otherwise.  No counterpart in the
+ * SLES_TUNE_XCD and SLES_TUNE_FUSE_X; SLES_TUNE_FFT_WIDE 3 at N = 512,
+ * 1 (FP64) / 2 (FP32) at N = 256, 0 otherwise.  No counterpart in the
  * reference.
  *   SLES_TUNE_STRIDED_RADIX4  x/y inverse + y forward passes use the radix-4
  *                             line plan (4 points per thread) instead of radix-8
@@ -53,6 +53,10 @@ int sles_plan_info(sles_plan plan, int* n, int* nh, int* precision);
  *   SLES_TUNE_Z_RADIX4        same for the fused z kernel
  *   SLES_TUNE_XCD             XCD-aware tile order in the strided passes (fused
  *                             step and the generic transforms)
+ *   SLES_TUNE_FUSE_X          RK4 x-forward pass also runs the next stage's
+ *                             x-inverse (1, default) or not (0); the fused
+ *                             kernels exist in FP32 and for FP64 N <= 64
+ *                             (ignored otherwise)
  *   SLES_TUNE_FFT_WIDE        generic transforms (sles_fft_r2c / sles_fft_c2r):
  *                             bit 0 = x pass, bit 1 = y pass run tiles of
  *                             twice the columns (256- or 512-byte row runs, LDS tile
@@ -64,7 +68,8 @@ enum {
   SLES_TUNE_XFWD_RADIX4 = 1,
   SLES_TUNE_Z_RADIX4 = 2,
   SLES_TUNE_XCD = 3,
-  SLES_TUNE_FFT_WIDE = 4
+  SLES_TUNE_FFT_WIDE = 4,
+  SLES_TUNE_FUSE_X = 5
 };
 int sles_plan_set_tuning(sles_plan plan, int key, int value);
 int sles_plan_get_tuning(sles_plan plan, int key, int* value);

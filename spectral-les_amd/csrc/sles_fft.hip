@@ -418,6 +418,7 @@ int sles_plan_set_tuning(sles_plan p, int key, int value) {
     case SLES_TUNE_XFWD_RADIX4: p->tune_xfwd = value != 0; return 0;
     case SLES_TUNE_Z_RADIX4: p->tune_z = value != 0; return 0;
     case SLES_TUNE_XCD: p->tune_xcd = value != 0; return 0;
+    case SLES_TUNE_FUSE_X: p->tune_fuse_x = value != 0; return 0;
     case SLES_TUNE_FFT_WIDE:
       if (value < 0 || value > 3) return fail("sles_plan_set_tuning: SLES_TUNE_FFT_WIDE takes 0..3");
       p->tune_fft_wide = value;
@@ -434,6 +435,7 @@ int sles_plan_get_tuning(sles_plan p, int key, int* value) {
     case SLES_TUNE_Z_RADIX4: *value = p->tune_z; return 0;
     case SLES_TUNE_XCD: *value = p->tune_xcd; return 0;
     case SLES_TUNE_FFT_WIDE: *value = p->tune_fft_wide; return 0;
+    case SLES_TUNE_FUSE_X: *value = p->tune_fuse_x; return 0;
     default: return fail("sles_plan_get_tuning: unknown key");
   }
 }

@@ -19,6 +19,7 @@ struct sles_plan_s {
   int tune_xfwd = 0;     // x forward + stage update
   int tune_z = 0;        // fused z kernel
   int tune_xcd = 1;      // XCD-aware tile order in the strided passes (default on)
+  int tune_fuse_x = 1;   // RK4 x-forward runs the next stage's x-inverse where compiled
   int tune_fft_wide = 0; // generic transforms: wide tiles in the x (bit 0) / y (bit 1) pass
                          // (default set per size in sles_plan_create)
 };

@@ -935,10 +935,10 @@ struct Fused {
   using SS = WideShape<T, N, PS>;  // x/y inverse and y forward
   using SX = XShape<T, N, PX>;  // x forward + stage update
   static constexpr int ZB = (int)(((long)N * N + ZLines<PZ>::LPB - 1) / ZLines<PZ>::LPB);
-  // RK4 x-forward runs the next stage's x-inverse (see k_xfwd): FP32 form, and
-  // FP64 grids up to 64^3 (launch-bound: one launch fewer per stage; 64^3
-  // Taylor-Green 0.323 -> 0.312 ms per step; at 128^3 FP64 the fused form
-  // was 2 % slower)
+  // RK4 x-forward can run the next stage's x-inverse (see k_xfwd; plan knob
+  // tune_fuse_x): FP32 form, and FP64 grids up to 64^3 (launch-bound: one
+  // launch fewer per stage; 64^3 Taylor-Green 0.323 -> 0.312 ms per step; at
+  // 128^3 FP64 the fused form was 2 % slower)
   static constexpr bool FUSE_X = sizeof(T) == 4 || N <= 64;
 
   // skip_xinv: W0-5 already hold the x-inverse output (fused into the previous x-forward)
@@ -1062,7 +1062,7 @@ int step_impl(const sles_plan_s* p, int scheme, const sles_step_params* prm, voi
     a.ca = rk4a[st];
     a.cb = rk4b[st];
     double* pp = (ddt && st == 0) ? part : nullptr;
-    const bool fuse = F::FUSE_X && scheme == SLES_SCHEME_RK4;
+    const bool fuse = F::FUSE_X && p->tune_fuse_x && scheme == SLES_SCHEME_RK4;
     const bool fused_in = fuse && st > 0;  // x-inverse ran in the previous x-forward
     if (F::stage(p, les, coeff, uh, W, pp, s, fused_in)) return -1;
     if (pp) {

@@ -28,7 +28,8 @@ import torch
 from . import _lib
 
 TABLE = os.environ.get("SLES_TUNING_TABLE", os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning.json"))
-KNOBS = ("strided_radix4", "xfwd_radix4", "z_radix4", "xcd")
+KNOBS = ("strided_radix4", "xfwd_radix4", "z_radix4", "xcd", "fuse_x")
+DEFAULTS = dict(strided_radix4=0, xfwd_radix4=0, z_radix4=0, xcd=1, fuse_x=1)  # a new plan's settings
 RADIX4_SIZES = (32, 64, 128, 256, 512)
 
 
@@ -53,10 +54,17 @@ def apply(plan, N, precision, les_model, path=TABLE):
     return entry["settings"]
 
 
-def candidates(N):
+def has_fused_x(N, precision):
+    """The fused x-forward + next x-inverse kernels are built for FP32 and for
+    FP64 grids up to 64^3."""
+    return precision == "fp32" or N <= 64
+
+
+def candidates(N, precision=None):
     radix = (0, 1) if N in RADIX4_SIZES else (0,)
-    for s, x, z, c in itertools.product(radix, radix, radix, (0, 1)):
-        yield dict(strided_radix4=s, xfwd_radix4=x, z_radix4=z, xcd=c)
+    fuse = (0, 1) if precision is None or has_fused_x(N, precision) else (1,)
+    for s, x, z, c, f in itertools.product(radix, radix, radix, (0, 1), fuse):
+        yield dict(strided_radix4=s, xfwd_radix4=x, z_radix4=z, xcd=c, fuse_x=f)
 
 
 def time_variant(domain, settings, steps=3, warmup=1):
@@ -85,7 +93,7 @@ def tune(N, precision="fp64", les="smagorinsky", steps=3, path=TABLE, verbose=Tr
     d = InitializeDomain(N=N, nu=1e-4, method="RK4", precision=precision, tuned=False, **kw)
     set_spectral_state(d, isotropic_spectrum(N, seed=0, dtype=d.cdtype))
     results = []
-    for st in candidates(N):
+    for st in candidates(N, precision):
         ms = time_variant(d, st, steps=steps)
         results.append((ms, st))
         if verbose:
@@ -96,7 +104,7 @@ def tune(N, precision="fp64", les="smagorinsky", steps=3, path=TABLE, verbose=Tr
     table[key(N, precision, d.sgs_model)] = {
         "settings": best,
         "ms_per_step": round(best_ms, 4),
-        "default_ms_per_step": round(next(ms for ms, st in results if not any(st.values())), 4),
+        "default_ms_per_step": round(next(ms for ms, st in results if st == DEFAULTS), 4),
         "device": torch.cuda.get_device_name(),
         "tuned_at": time.strftime("%Y-%m-%d %H:%M:%S"),
     }
