@@ -62,7 +62,9 @@ int sles_plan_info(sles_plan plan, int* n, int* nh, int* precision);
  *                             twice the columns (256- or 512-byte row runs, LDS tile
  *                             up to 128 KiB); N = 256 and 512 only (ignored
  *                             otherwise); values 0..3
- * The radix-4 plans exist for N = 32, 64, 128, 256, 512 (ignored otherwise). */
+ * The "radix-4" alternative plans: 4 points per thread (radix 4) for
+ * N = 32, 64, 128, 256, 512; 6 points per thread (radix 3, then 2) for
+ * N = 12, 24, 48, 96, 192, 384; ignored for N = 8, 16. */
 enum {
   SLES_TUNE_STRIDED_RADIX4 = 0,
   SLES_TUNE_XFWD_RADIX4 = 1,

@@ -125,8 +125,12 @@ SLES_PLAN(384, 12, 4, 4, 4, 3, 2)
 SLES_PLAN(512, 8, 8, 8, 8)
 #undef SLES_PLAN
 
-// Radix-4 plans (4 points per thread: half the registers of the radix-8
-// plans, twice the threads per line, more LDS exchanges).
+// Alternative plans with fewer points per thread (selected per pass by the
+// tuning knobs): radix-4 (4 points per thread: half the registers of the
+// radix-8 plans, twice the threads per line, more LDS exchanges) for powers
+// of two, and radix 3 then radix 2 (6 points per thread, half the registers
+// of the 12-point plans, which spill in the fused passes) for the sizes with
+// a factor 3.
 template <int N>
 struct LinePlan4 : LinePlan<N> {};
 #define SLES_PLAN4(NN, ...) \
@@ -138,6 +142,16 @@ SLES_PLAN4(128, 4, 4, 4, 2)
 SLES_PLAN4(256, 4, 4, 4, 4)
 SLES_PLAN4(512, 4, 4, 4, 4, 2)
 #undef SLES_PLAN4
+#define SLES_PLAN6(NN, ...) \
+  template <>               \
+  struct LinePlan4<NN> : Plan<NN, 6, __VA_ARGS__> {};
+SLES_PLAN6(12, 3, 2, 2)
+SLES_PLAN6(24, 3, 2, 2, 2)
+SLES_PLAN6(48, 3, 2, 2, 2, 2)
+SLES_PLAN6(96, 3, 2, 2, 2, 2, 2)
+SLES_PLAN6(192, 3, 2, 2, 2, 2, 2, 2)
+SLES_PLAN6(384, 3, 2, 2, 2, 2, 2, 2, 2)
+#undef SLES_PLAN6
 
 // ---------------------------------------------------------------------------
 // Radix-R DFT in registers.  DIR = -1 forward (e^{-2 pi i nk/R}), +1 inverse.

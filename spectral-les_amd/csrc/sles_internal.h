@@ -78,7 +78,7 @@ struct ZShape {
 };
 
 // Strided passes: TK columns per block, LDS tile N*TK complex <= 64 KiB.
-// Blocks of plans with >= 8 points per thread are capped at 512 threads (8
+// Blocks of plans with >= 6 points per thread are capped at 512 threads (8
 // waves: up to 256 VGPRs per wave without spilling); radix-4 plans (4 points
 // per thread, about half the registers) may use 1024.  MAXT_ > 0 overrides
 // the cap for one kernel.
@@ -86,7 +86,7 @@ template <typename T, int N, class PL = LinePlan<N>, int MAXT_ = 0>
 struct StridedShape {
   static constexpr int P = PL::P;
   static constexpr int NT = N / P;
-  static constexpr int MAXT = MAXT_ > 0 ? MAXT_ : (P >= 8 ? 512 : 1024);
+  static constexpr int MAXT = MAXT_ > 0 ? MAXT_ : (P >= 6 ? 512 : 1024);
   static constexpr int TK0 = 65536 / (N * (int)sizeof(cpx<T>));
   static constexpr int TK1 = TK0 > 64 ? 64 : TK0;
   static constexpr int TK = (TK1 * NT > MAXT) ? (MAXT / NT > 0 ? MAXT / NT : 1) : TK1;

@@ -1087,7 +1087,9 @@ int step_impl(const sles_plan_s* p, int scheme, const sles_step_params* prm, voi
 // the radix-4 plan where one exists for N.
 template <int N>
 struct HasP4 {
-  static constexpr bool value = (N == 32 || N == 64 || N == 128 || N == 256 || N == 512);
+  // sizes with an alternative plan (LinePlan4)
+  static constexpr bool value = (N == 32 || N == 64 || N == 128 || N == 256 || N == 512 || N == 12 || N == 24 ||
+                                 N == 48 || N == 96 || N == 192 || N == 384);
 };
 
 template <typename T, int N>

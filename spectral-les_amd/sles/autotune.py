@@ -30,7 +30,7 @@ from . import _lib
 TABLE = os.environ.get("SLES_TUNING_TABLE", os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning.json"))
 KNOBS = ("strided_radix4", "xfwd_radix4", "z_radix4", "xcd", "fuse_x")
 DEFAULTS = dict(strided_radix4=0, xfwd_radix4=0, z_radix4=0, xcd=1, fuse_x=1)  # a new plan's settings
-RADIX4_SIZES = (32, 64, 128, 256, 512)
+RADIX4_SIZES = (12, 24, 32, 48, 64, 96, 128, 192, 256, 384, 512)  # sizes with an alternative line plan
 
 
 def key(N, precision, les_model):

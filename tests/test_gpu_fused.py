@@ -152,13 +152,13 @@ def test_taylor_green_64_rk4_trajectory():
     assert rel(g.U_hat, o.U_hat) < 1e-10
 
 
+@pytest.mark.parametrize("N", [32, 48])  # 48: 12-point plan and its 6-point alternative
 @pytest.mark.parametrize("les", [None, "Smagorinsky", "Vreman"])
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
-def test_every_tuning_variant_matches_oracle(les, precision):
+def test_every_tuning_variant_matches_oracle(les, precision, N):
     """Each kernel variant the autotuner may pick gives the oracle's step."""
     from sles import autotune, fused
 
-    N = 32
     tol = 1e-11 if precision == "fp64" else 1e-5
     g, o, u = twins(N, seed=14, fused=True, precision=precision,
                     **_kw(les, nu=0.005, time_stepSize=0.01, method="RK4"))
