@@ -750,8 +750,9 @@ int adjoint_t(const sles_plan_s* p, const sles_step_params* prm, const void* y, 
   using D = LinePlan<N>;
   if constexpr (HasP4<N>::value) {
     using Q = LinePlan4<N>;
-    // the adjoint z kernel holds 21 physical lines per thread: radix-4 plan
-    // (half the registers of radix-8) in both precisions
+    // the adjoint z kernel holds 21 physical lines per thread: the
+    // alternative plan (radix-4, or 6 points for sizes with a factor 3; half
+    // the registers of the default plan) in both precisions
     if (p->tune_strided) return adjoint_impl<T, N, Q, Q>(p, prm, y, mu, out, accumulate, linear, dcoeff, cs_grad, work, s);
     return adjoint_impl<T, N, D, Q>(p, prm, y, mu, out, accumulate, linear, dcoeff, cs_grad, work, s);
   }

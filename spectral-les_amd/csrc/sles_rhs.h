@@ -1083,8 +1083,9 @@ int step_impl(const sles_plan_s* p, int scheme, const sles_step_params* prm, voi
   return 0;
 }
 
-// Line plans selectable per kernel group (plan->tune_*): radix-8 default or
-// the radix-4 plan where one exists for N.
+// Line plans selectable per kernel group (plan->tune_*): the default plan or
+// the alternative one (LinePlan4: radix-4, or 6 points per thread for the
+// sizes with a factor 3) where one exists for N.
 template <int N>
 struct HasP4 {
   // sizes with an alternative plan (LinePlan4)
