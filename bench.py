@@ -75,12 +75,13 @@ def xfwd_mean_fields(les, fused=False):
     stages 1-2 read W + U_hat, U_hat0, U_hat1 and write U_hat1, U_hat;
     stage 3 reads W + U_hat, U_hat1 and writes U_hat1, U_hat.
     fused (FP32 form): stages 0-2 also write the next stage's x-inverse
-    output (6 fields with LES, 5 without), which then reads nothing more."""
+    output (6 fields with LES, 5 without), stage 3 the physical-output
+    pass's (5 fields); those passes then read nothing more."""
     w = 6 if les else 3
     per_stage = [w + 3 + 9, w + 9 + 6, w + 9 + 6, w + 6 + 6]
     if fused:
         xi = 6 if les else 5
-        per_stage = [f + xi for f in per_stage[:3]] + per_stage[3:]
+        per_stage = [f + xi for f in per_stage[:3]] + [per_stage[3] + 5]
     return sum(per_stage) / 4.0
 
 

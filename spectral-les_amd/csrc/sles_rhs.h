@@ -697,6 +697,7 @@ __device__ __forceinline__ void epilogue(cpx<T> (&s)[3], eidx e, int kx, int ky,
         const cpx<T> fin = ldx(u1i, e) + ca * d[i];
         stx(u1i, e, fin);
         stx(ui, e, fin);
+        ynext[i] = fin;  // for the physical-output x-inverse
       }
     }
     return;
@@ -735,7 +736,10 @@ using XShape = StridedShape<T, N, PL, (sizeof(T) == 4 && PL::P == 4) ? 1024 : 51
 // the fused inverse runs on the x-forward's radix-4 plan at 2 waves per SIMD
 // and was slower than the separate radix-8 k_xinv (1.071 against 1.038 ms),
 // so FP64 keeps the two kernels except on launch-bound grids (Fused::FUSE_X).
-template <typename T, int N, int LES, class PL, bool FUSE = false>
+// FLES: layout of the fused x-inverse output (LES stage layout, or 0: the
+// layout without LES, which is also the physical-output pass's: the last RK4
+// stage runs that pass's x-inverse on the new U_hat).
+template <typename T, int N, int LES, class PL, bool FUSE = false, int FLES = LES>
 __global__ void __launch_bounds__((XShape<T, N, PL>::THREADS))
     k_xfwd(cpx<T>* __restrict__ W, long C, cpx<T>* uh, cpx<T>* uh0, cpx<T>* uh1, cpx<T>* aux, StepArgs a,
            const cpx<T>* __restrict__ tw, int xcd) {
@@ -745,9 +749,13 @@ __global__ void __launch_bounds__((XShape<T, N, PL>::THREADS))
   using FFT = LineFFTP<T, PL, -1, false, PPB>;
   __shared__ cpx<T> lds[N * TK + PPB];
   double dt = a.dt;
+  bool skip = false;  // the step is not taken (device dt NaN or too small)
   if (a.dt_dev) {
     dt = *a.dt_dev;
-    if (!(dt >= 1e-6)) return;  // NaN or too small: the step is not taken
+    if (!(dt >= 1e-6)) {
+      if constexpr (!FUSE) return;
+      skip = true;  // fused: U_hat stays; the x-inverse below still runs on it
+    }
   }
   const Cols<N, TK> col(xcd);
   const int t = threadIdx.x / TK;
@@ -759,11 +767,12 @@ __global__ void __launch_bounds__((XShape<T, N, PL>::THREADS))
   cpx<T> a0[P], a1[P], a2[P], v[P];
   // next field's loads are issued before the current field's transform
   constexpr int NF = LES ? 6 : 3;
+  const int nf = skip ? 0 : NF;
   cpx<T> nx[P];
 #pragma unroll
   for (int m = 0; m < P; ++m) nx[m] = ldx(W, at(m));
 #pragma unroll 1
-  for (int f = 0; f < NF; ++f) {
+  for (int f = 0; f < nf; ++f) {
 #pragma unroll
     for (int m = 0; m < P; ++m) v[m] = nx[m];
     if (f + 1 < NF) {
@@ -803,7 +812,12 @@ __global__ void __launch_bounds__((XShape<T, N, PL>::THREADS))
   for (int m = 0; m < P; ++m) {
     cpx<T> s3[3] = {a0[m], a1[m], a2[m]};
     cpx<T> yn[3] = {{T(0), T(0)}, {T(0), T(0)}, {T(0), T(0)}};
-    if (col.valid) epilogue<T>(s3, at(m), kfreq(t + m * NT, N), ky, kz, N, a, dt, uh, uh0, uh1, aux, C, yn);
+    if (col.valid && !skip) {
+      epilogue<T>(s3, at(m), kfreq(t + m * NT, N), ky, kz, N, a, dt, uh, uh0, uh1, aux, C, yn);
+    } else if (col.valid) {
+#pragma unroll
+      for (int i = 0; i < 3; ++i) yn[i] = ldx(uh + i * C, at(m));
+    }
     y0[m] = yn[0];
     y1[m] = yn[1];
     y2[m] = yn[2];
@@ -817,7 +831,7 @@ __global__ void __launch_bounds__((XShape<T, N, PL>::THREADS))
 #pragma unroll
         for (int m = 0; m < P; ++m) stx(W + j * C, at(m), x[m]);
     };
-    if constexpr (LES) {
+    if constexpr (FLES) {
       // as k_xinv: W_j = F_x^-1 Y_j, W_{3+j} = F_x^-1 (1j Kx Y_j)
 #pragma unroll
       for (int m = 0; m < P; ++m) v[m] = mul_i(kx(m) * y0[m]);
@@ -976,8 +990,9 @@ struct Fused {
   }
 
   // fuse_next: also run the next stage's x-inverse (RK4 stages 0-2)
+  // fuse_phys (last RK4 stage): also run the physical-output pass's x-inverse
   static int finish(const sles_plan_s* p, int les, cpx<T>* uh, cpx<T>* uh0, cpx<T>* uh1, cpx<T>* aux, cpx<T>* W,
-                    const StepArgs& a, hipStream_t s, bool fuse_next = false) {
+                    const StepArgs& a, hipStream_t s, bool fuse_next = false, bool fuse_phys = false) {
     const long C = ncplx(p);
     const cpx<T>* tw = (const cpx<T>*)p->tw;
     const int xcd = p->tune_xcd;
@@ -987,17 +1002,22 @@ struct Fused {
     if (les) {
       { ProfScope ps_(PK_YFWD, s); hipLaunchKernelGGL((k_yfwd<T, N, 1, PS>), gs, bs, 0, s, W, C, tw, xcd); }
       ProfScope ps_(PK_XFWD, s);
-      if constexpr (FUSE_X)
+      if constexpr (FUSE_X) {
         if (fuse_next) {
           hipLaunchKernelGGL((k_xfwd<T, N, 1, PX, true>), gx, bx, 0, s, W, C, uh, uh0, uh1, aux, a, tw, xcd);
           return check_launch("fused stage (forward)");
         }
+        if (fuse_phys) {
+          hipLaunchKernelGGL((k_xfwd<T, N, 1, PX, true, 0>), gx, bx, 0, s, W, C, uh, uh0, uh1, aux, a, tw, xcd);
+          return check_launch("fused stage (forward)");
+        }
+      }
       hipLaunchKernelGGL((k_xfwd<T, N, 1, PX>), gx, bx, 0, s, W, C, uh, uh0, uh1, aux, a, tw, xcd);
     } else {
       { ProfScope ps_(PK_YFWD, s); hipLaunchKernelGGL((k_yfwd<T, N, 0, PS>), gs, bs, 0, s, W, C, tw, xcd); }
       ProfScope ps_(PK_XFWD, s);
       if constexpr (FUSE_X)
-        if (fuse_next) {
+        if (fuse_next || fuse_phys) {  // the same output layout
           hipLaunchKernelGGL((k_xfwd<T, N, 0, PX, true>), gx, bx, 0, s, W, C, uh, uh0, uh1, aux, a, tw, xcd);
           return check_launch("fused stage (forward)");
         }
@@ -1006,14 +1026,16 @@ struct Fused {
     return check_launch("fused stage (forward)");
   }
 
-  static int physical(const sles_plan_s* p, cpx<T>* uh, cpx<T>* W, void* u_out, void* curl_out, hipStream_t s) {
+  // skip_xinv: W0-4 already hold the x-inverse output (fused into the last x-forward)
+  static int physical(const sles_plan_s* p, cpx<T>* uh, cpx<T>* W, void* u_out, void* curl_out, hipStream_t s,
+                      bool skip_xinv = false) {
     const long C = ncplx(p);
     const cpx<T>* tw = (const cpx<T>*)p->tw;
     const int xcd = p->tune_xcd;
     constexpr long NCOL = (long)N * (N / 2 + 1);  // columns of every strided pass
     const dim3 gs((unsigned)((NCOL + SS::TK - 1) / SS::TK)), bs(SS::THREADS);
     ProfScope ps_(PK_PHYS, s);
-    hipLaunchKernelGGL((k_xinv<T, N, 0, PS>), gs, bs, 0, s, uh, W, C, tw, xcd);
+    if (!skip_xinv) hipLaunchKernelGGL((k_xinv<T, N, 0, PS>), gs, bs, 0, s, uh, W, C, tw, xcd);
     hipLaunchKernelGGL((k_yinv<T, N, 0, PS>), gs, bs, 0, s, W, C, tw, xcd);
     if (check_launch("fused physical")) return -1;
     const double sc = 1.0 / ((double)N * N * N);
@@ -1077,9 +1099,12 @@ int step_impl(const sles_plan_s* p, int scheme, const sles_step_params* prm, voi
                          scalars);
       if (check_launch("cfl")) return -1;
     }
-    if (F::finish(p, les, uh, uh0, uh1, aux, W, a, s, fuse && st < 3)) return -1;
+    if (F::finish(p, les, uh, uh0, uh1, aux, W, a, s, fuse && st < 3, fuse && st == 3 && (u_out || curl_out)))
+      return -1;
   }
-  if (u_out || curl_out) return F::physical(p, uh, W, u_out, curl_out, s);
+  // the last fused RK4 x-forward already ran the physical pass's x-inverse
+  const bool phys_fused = F::FUSE_X && p->tune_fuse_x && scheme == SLES_SCHEME_RK4;
+  if (u_out || curl_out) return F::physical(p, uh, W, u_out, curl_out, s, phys_fused);
   return 0;
 }
 

@@ -104,15 +104,25 @@ def test_fused_device_dt_matches_cfl_rule(les, N):
     assert rel(g.U_hat, o.U_hat) < 1e-11
 
 
-def test_fused_device_dt_guard_skips_step():
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("les", [None, "Smagorinsky"])
+@pytest.mark.parametrize("fuse_x", [0, 1])
+def test_fused_device_dt_guard_skips_step(precision, les, fuse_x):
+    """The skipped step leaves U_hat as it was and writes U and curl of it,
+    also when the last x-forward runs the physical pass's x-inverse."""
     from sles import fused
 
-    g, o, u = twins(16, seed=9, fused=True, nu=0.01, method="RK4")
+    g, o, u = twins(16, seed=9, fused=True, precision=precision, **_kw(les, nu=0.01, method="RK4"))
+    g.plan.set_tuning(fuse_x=fuse_x)
     before = g.U_hat.clone()
+    U0, curl0 = torch.stack(g.U).clone(), g.curl.clone()
     # a tiny CFL number forces dt < 1e-6: the step must not be taken
     fused.rk4_step(g, dt_device=True, cfl=1e-9)
     assert g._scalars[1].item() < 1e-6
     assert torch.equal(g.U_hat, before)
+    tol = 1e-12 if precision == "fp64" else 1e-5
+    assert rel(torch.stack(g.U), U0) < tol
+    assert rel(g.curl, curl0) < tol
 
 
 @pytest.mark.parametrize("les", [None, "Smagorinsky", "Vreman"])
