@@ -1,4 +1,4 @@
-"""Differentiable FP64 restatement of the RHS + RK4 / ETD2 step in torch (CPU),
+"""Differentiable FP64 restatement of the RHS + RK4 / ETD2 steps in torch (CPU),
 used only by the adjoint tests as the autograd reference.
 
 Same algebra as oracle/sles_oracle.py (computeRHS, src/RK4_method.py:38-71
@@ -101,6 +101,45 @@ def rk4_step(Uh, Cs, N, nu, dt, model=None, forcing=0.0):
             Y = U0 + b[s] * dt * k
         U1 = U1 + a[s] * dt * k
     return U1
+
+
+def phi1(z):
+    """phi1(z) = (e^z - 1)/z, FP64; Taylor series for |z| < 1e-2 (z = -h nu K^2 <= 0)."""
+    small = z.abs() < 1e-2
+    zs = torch.where(small, z, torch.zeros_like(z))
+    zl = torch.where(small, -torch.ones_like(z), z)
+    taylor = 1 + zs / 2 * (1 + zs / 3 * (1 + zs / 4 * (1 + zs / 5 * (1 + zs / 6))))
+    return torch.where(small, taylor, torch.expm1(zl) / zl)
+
+
+def phi2(z):
+    """phi2(z) = (e^z - 1 - z)/z^2, FP64; Taylor series for |z| < 1e-2."""
+    small = z.abs() < 1e-2
+    zs = torch.where(small, z, torch.zeros_like(z))
+    zl = torch.where(small, -torch.ones_like(z), z)
+    taylor = 0.5 * (1 + zs / 3 * (1 + zs / 4 * (1 + zs / 5 * (1 + zs / 6 * (1 + zs / 7)))))
+    return torch.where(small, taylor, (torch.expm1(zl) - zl) / (zl * zl))
+
+
+def etd2_run(Uh, Cs, N, nu, dt, nsteps, model=None, forcing=0.0):
+    """nsteps of the product ETD2 from Uh: the first step is an RK4 step (as
+    src/ETD2_method.py:9-11), then Cox-Matthews ETD2 in phi form
+        U_{n+1} = e^{hL} U_n + h (phi1 + phi2)(hL) F(U_n) - h phi2(hL) F(U_{n-1}),
+    L = -nu K^2, with F the whole RHS but the viscous term (SGS stress and
+    forcing included) and F(U_{-1}) := F(U_0) for the first ETD2 step.  The
+    phi functions are evaluated in FP64 and hold no dependence on U or Cs."""
+    K = wavenumbers(N)
+    z = -dt * nu * (K * K).sum(0)
+    E = torch.exp(z)
+    w1 = dt * (phi1(z) + phi2(z))
+    w0 = dt * phi2(z)
+    F_prev = rhs(Uh, Cs, N, nu, model, forcing, viscous=False)
+    U = rk4_step(Uh, Cs, N, nu, dt, model, forcing)
+    for _ in range(1, nsteps):
+        Fn = rhs(U, Cs, N, nu, model, forcing, viscous=False)
+        U = E * U + w1 * Fn - w0 * F_prev
+        F_prev = Fn
+    return U
 
 
 def shell_spectrum(Uh, N, kmax=40):
