@@ -226,6 +226,23 @@ typedef struct {
 
 size_t sles_step_work_bytes(sles_plan plan, int les_model);
 
+/* sles_step without a workspace argument, for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.  The
+ * workspace is a static device array of SLES_STEP_SCRATCH_BYTES (64 MiB)
+ * that every device running the library holds until the process ends; its
+ * address is taken per device by sles_plan_create.  Calls whose workspace
+ * (sles_step_work_bytes) exceeds it are rejected with an error: with LES up
+ * to N = 96 in FP64 and N = 96 in FP32 (128 in FP32 needs 73 MiB), without
+ * LES up to 96 (FP64) and 128 (FP32).  Every call on a device shares that
+ * device's array, so calls that overlap on one device (two streams, two
+ * threads) race; each call writes the array before it reads it, so nothing
+ * carries over between calls.  Same arguments, results and errors as
+ * sles_step otherwise; sles_profile_enable does not time it. */
+#define SLES_STEP_SCRATCH_BYTES (64u << 20)
+int sles_step_noalloc(sles_plan plan, int scheme, const sles_step_params* params, void* u_hat, void* u_hat0,
+                      void* u_hat1, void* aux, void* u_out, void* curl_out, double* scalars, void* stream);
+
 /* Kernel timing of sles_step for measurement: while enabled, every launch of
  * sles_step is bracketed by hipEvents on its stream.  sles_profile_read
  * synchronises on them, adds the elapsed milliseconds and launch counts per

@@ -369,6 +369,7 @@ int sles_plan_create(sles_plan* out, int n, int precision) {
   // (FP32) at 256
   p->tune_fft_wide = n == 512 ? 3 : n == 256 ? (precision == SLES_FP64 ? 1 : 2) : 0;
   (void)hipGetDevice(&p->device);
+  step_scratch_init(p->device);
   // Twiddles exp(-2 pi i e/n) evaluated in long double and rounded once.
   const size_t esz = precision == SLES_FP64 ? 16 : 8;
   std::vector<unsigned char> host(esz * n);

@@ -100,6 +100,10 @@ struct StridedShape {
 template <typename T, int N, class PL = LinePlan<N>>
 using WideShape = StridedShape<T, N, PL, (sizeof(T) == 4 && PL::NT >= 64) ? 1024 : 0>;
 
+// Takes the address of the no-allocation step's device scratch on `device`
+// (sles_rhs.hip); sles_plan_create calls it with that device current.
+void step_scratch_init(int device);
+
 int z_c2r_rows(const sles_plan_s* p, const void* in, void* out, long nrows, double scale, hipStream_t s);
 
 }  // namespace sles

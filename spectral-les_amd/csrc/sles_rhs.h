@@ -911,6 +911,7 @@ struct ProfRec {
   hipEvent_t a, b;
 };
 extern bool g_prof;
+extern thread_local bool g_prof_off;  // set while sles_step_noalloc runs: no timing events
 extern std::vector<ProfRec> g_recs;
 extern std::vector<hipEvent_t> g_pool;
 
@@ -930,7 +931,7 @@ struct ProfScope {
   hipStream_t s;
   hipEvent_t a = nullptr;
   ProfScope(int k, hipStream_t st) : kind(k), s(st) {
-    if (g_prof && (a = prof_event())) (void)hipEventRecord(a, s);
+    if (g_prof && !g_prof_off && (a = prof_event())) (void)hipEventRecord(a, s);
   }
   ~ProfScope() {
     if (!a) return;

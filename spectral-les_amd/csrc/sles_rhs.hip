@@ -1,11 +1,56 @@
 // C ABI of the fused time step (kernels in sles_rhs.h, instantiated per
 // precision in sles_rhs_f64.hip / sles_rhs_f32.hip) and its kernel timing.
+#include <mutex>
+
 #include "sles_rhs.h"
+
+// Workspace of sles_step_noalloc: the fused step's W fields and CFL partials
+// for callers that cannot allocate device memory.  Each device that runs this
+// library holds its own copy until the process ends; host code reaches it
+// only through the address that step_scratch_init takes per device.
+static __device__ __align__(256) unsigned char scratch[SLES_STEP_SCRATCH_BYTES];
 
 namespace sles {
 bool g_prof = false;
+thread_local bool g_prof_off = false;
 std::vector<ProfRec> g_recs;
 std::vector<hipEvent_t> g_pool;
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+std::string g_scratch_err[kMaxDevices];
+}  // namespace
+
+// Device address of `scratch` on `device`, which must be the current device.
+// Called once per device from sles_plan_create (the address differs per
+// device); a failure is kept and reported by sles_step_noalloc.
+void step_scratch_init(int device) {
+  if (device < 0 || device >= kMaxDevices) return;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[device]) return;
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr) {
+    g_scratch_err[device] = std::string("hipGetSymbolAddress: ") + hipGetErrorString(e);
+    (void)hipGetLastError();
+    return;
+  }
+  g_scratch_ptr[device] = ptr;
+  g_scratch_err[device].clear();
+}
+
+static void* step_scratch(int device, std::string* why) {
+  if (device < 0 || device >= kMaxDevices) {
+    *why = "device index out of range";
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[device])
+    *why = g_scratch_err[device].empty() ? std::string("not initialised") : g_scratch_err[device];
+  return g_scratch_ptr[device];
+}
 }  // namespace sles
 
 using namespace sles;
@@ -36,6 +81,25 @@ int sles_step(sles_plan p, int scheme, const sles_step_params* prm, void* u_hat,
   return p->precision == SLES_FP64
              ? step_f64(p, scheme, prm, u_hat, u_hat0, u_hat1, aux, u_out, curl_out, scalars, work, s, nullptr)
              : step_f32(p, scheme, prm, u_hat, u_hat0, u_hat1, aux, u_out, curl_out, scalars, work, s, nullptr);
+}
+
+int sles_step_noalloc(sles_plan p, int scheme, const sles_step_params* prm, void* u_hat, void* u_hat0,
+                      void* u_hat1, void* aux, void* u_out, void* curl_out, double* scalars, void* stream) {
+  if (!p || !prm) return fail("sles_step_noalloc: null argument");
+  const size_t need = sles_step_work_bytes(p, prm->les_model);
+  if (need > (size_t)SLES_STEP_SCRATCH_BYTES)
+    return fail("sles_step_noalloc: the step needs " + std::to_string(need) + " bytes of workspace, above the " +
+                std::to_string((size_t)SLES_STEP_SCRATCH_BYTES) + "-byte device scratch (N = " +
+                std::to_string(p->n) + "); use sles_step with a caller workspace");
+  std::string why;
+  void* work = step_scratch(p->device, &why);
+  if (!work) return fail("sles_step_noalloc: device scratch unavailable on device " + std::to_string(p->device) +
+                         " (" + why + ")");
+  // no kernel-timing events here: sles_profile_enable would create them
+  g_prof_off = true;
+  const int rc = sles_step(p, scheme, prm, u_hat, u_hat0, u_hat1, aux, u_out, curl_out, scalars, work, stream);
+  g_prof_off = false;
+  return rc;
 }
 
 int sles_rk4_stage_states(sles_plan p, const sles_step_params* prm, void* u_hat, void* u_hat0, void* u_hat1,
