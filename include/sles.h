@@ -226,6 +226,53 @@ typedef struct {
 
 size_t sles_step_work_bytes(sles_plan plan, int les_model);
 
+/* The driver loop of spectralMethodSolver (src/SM_solver.py:18-68) on the
+ * device: nsteps fused steps of `scheme` (RK4, RK3 or ETD2; params->dt and
+ * dt_device are not used) with dt and the loop's stop rules evaluated per
+ * step on the device, so the host enqueues a chunk of steps with no
+ * synchronisation and reads `state` once afterwards.  Each step:
+ *   1. max(u) of the state (signed, as numpy.max of U) and
+ *      dt = min(CFL dx / max u, 0.1 dx^2 / nu), CFL = cfl_first until the
+ *      first step is taken (0.3 in the reference, :21-23), params->cfl after
+ *      (:60-63);
+ *   2. dt NaN: status NAN (the loop's break, :64-67); dt < 1e-6: status
+ *      DT_TOO_SMALL (compute_dt's exit, :273-276);
+ *   3. t = dt if state[T] is NaN (t starts at the first dt, :25);
+ *   4. loop top: t >= state[T_PAUSE] stops with status PAUSED so that the
+ *      host can write the outputs due at this t (:40-45), unless the status
+ *      on entry is RESUME (the host has written them); t >= state[T_FINAL]
+ *      or dt < 1e-8: status DONE (:46-47);
+ *   5. t += dt (:51), steps += 1, and the step is taken with dt.
+ * Once the status is not RUNNING (or RESUME) the remaining steps of the
+ * call leave U_hat unchanged.  The caller starts a run with t = NaN,
+ * status = RUNNING, steps = 0 and sets T_PAUSE (+inf for none) and T_FINAL;
+ * state holds SLES_DRIVE_STATE_SIZE doubles in device memory.  u_out /
+ * curl_out receive U and curl of the state after every step (the state the
+ * reference leaves), `work` as for sles_step.  No counterpart in the
+ * reference as an entry point (its loop is host Python). */
+enum {
+  SLES_DRIVE_STATE_UMAX = 0,     /* max u of the last state examined */
+  SLES_DRIVE_STATE_DT = 1,       /* dt from the CFL rule of that state */
+  SLES_DRIVE_STATE_T = 2,        /* the loop's t (NaN before the run) */
+  SLES_DRIVE_STATE_STATUS = 3,   /* SLES_DRIVE_* below */
+  SLES_DRIVE_STATE_STEPS = 4,    /* steps taken in this run */
+  SLES_DRIVE_STATE_DT_APPLY = 5, /* dt of the last step, NaN if not taken */
+  SLES_DRIVE_STATE_T_PAUSE = 6,  /* next output time (set by the host) */
+  SLES_DRIVE_STATE_T_FINAL = 7,  /* final time T (set by the host) */
+  SLES_DRIVE_STATE_SIZE = 8
+};
+enum {
+  SLES_DRIVE_RUNNING = 0,
+  SLES_DRIVE_DONE = 1,
+  SLES_DRIVE_NAN = 2,
+  SLES_DRIVE_DT_TOO_SMALL = 3,
+  SLES_DRIVE_PAUSED = 4,
+  SLES_DRIVE_RESUME = 5
+};
+int sles_step_drive(sles_plan plan, int scheme, const sles_step_params* params, double cfl_first, int nsteps,
+                    void* u_hat, void* u_hat0, void* u_hat1, void* aux, void* u_out, void* curl_out, double* state,
+                    void* work, void* stream);
+
 /* sles_step without a workspace argument, for callers that cannot allocate
  * device memory (code under HIP graph capture, tight serving loops,
  * harnesses that forbid hipMalloc).  No counterpart in the reference.  The

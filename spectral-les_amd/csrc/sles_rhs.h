@@ -38,6 +38,18 @@ namespace sles {
 
 enum { EPI_RK4 = 0, EPI_RK3 = 1, EPI_ETD2 = 2, EPI_RHS = 3, EPI_NONLIN = 4 };
 
+// Slots of the sles_step_drive state (the SLES_DRIVE_STATE_* layout of sles.h).
+enum {
+  DRV_UMAX = SLES_DRIVE_STATE_UMAX,
+  DRV_DT = SLES_DRIVE_STATE_DT,
+  DRV_T = SLES_DRIVE_STATE_T,
+  DRV_STATUS = SLES_DRIVE_STATE_STATUS,
+  DRV_STEPS = SLES_DRIVE_STATE_STEPS,
+  DRV_DT_APPLY = SLES_DRIVE_STATE_DT_APPLY,
+  DRV_T_PAUSE = SLES_DRIVE_STATE_T_PAUSE,
+  DRV_T_FINAL = SLES_DRIVE_STATE_T_FINAL
+};
+
 // Element index inside one spectral field and the accessors of the strided
 // passes.
 using eidx = long;
@@ -53,6 +65,13 @@ struct StepArgs {
   double ca, cb;  // RK4: a[s], b[s]
   int scheme, stage;
   void* stages;  // RK4: if set, stage s < 3 also writes its new state Y_{s+1} to stages + 3 s C
+};
+
+// sles_step_drive: the driver-loop state (device doubles, DRV_* layout) and
+// the CFL of the first step.
+struct DriveArgs {
+  double* state;
+  double cfl_first;
 };
 
 // ---------------------------------------------------------------------------
@@ -902,6 +921,69 @@ static __global__ void k_cfl(const double* part, int np, double cfl, double dx, 
   }
 }
 
+// The driver loop's per-step rules on the device (sles_step_drive; state
+// layout DRV_* in sles.h): dt from the CFL rule on max(u) of the state about
+// to be stepped (cfl_first until the first step is taken,
+// src/SM_solver.py:21-23,60-63), then the checks the reference loop makes
+// between two steps -- dt NaN ends the run (:64-67), dt < 1e-6 is the exit
+// of compute_dt (:273-276); t starts at the first dt (:25); a loop top with
+// t >= t_pause stops for the host's outputs (:40-45) unless the host resumed
+// this loop top; t >= t_final or dt < 1e-8 ends the run (:46-47) -- and
+// t += dt (:51).  state[DRV_DT_APPLY] receives the dt of the step, or NaN
+// when it is not taken (the x-forward kernels then leave U_hat unchanged).
+static __global__ void k_cfl_drive(const double* part, int np, double cfl, double cfl_first, double dx, double nu,
+                                   double* st) {
+  double m = -INFINITY;
+  for (int i = threadIdx.x; i < np; i += blockDim.x) m = (isnan(m) || m > part[i]) ? m : part[i];
+  __shared__ double s[1024];
+  s[threadIdx.x] = m;
+  block_sync();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) {
+      const double a = s[threadIdx.x], b = s[threadIdx.x + o];
+      s[threadIdx.x] = (isnan(a) || a > b) ? a : b;
+    }
+    block_sync();
+  }
+  if (threadIdx.x != 0) return;
+  const double umax = s[0];
+  st[DRV_UMAX] = umax;
+  const double status = st[DRV_STATUS];
+  const bool resume = status == (double)SLES_DRIVE_RESUME;
+  if (status != (double)SLES_DRIVE_RUNNING && !resume) {  // stopped earlier in this chunk
+    st[DRV_DT_APPLY] = NAN;
+    return;
+  }
+  const double c = st[DRV_STEPS] == 0.0 ? cfl_first : cfl;
+  const double dt1 = c * dx / umax;
+  const double dt2 = nu > 0 ? 0.1 * dx * dx / nu : 1e6;
+  const double dt = (dt2 < dt1) ? dt2 : dt1;
+  st[DRV_DT] = dt;
+  double stop = 0.0;
+  double t = st[DRV_T];
+  if (isnan(dt)) {
+    stop = SLES_DRIVE_NAN;
+  } else if (dt < 1e-6) {
+    stop = SLES_DRIVE_DT_TOO_SMALL;
+  } else {
+    if (isnan(t)) t = dt;
+    st[DRV_T] = t;
+    if (!resume && t >= st[DRV_T_PAUSE])
+      stop = SLES_DRIVE_PAUSED;
+    else if (t >= st[DRV_T_FINAL] || dt < 1e-8)
+      stop = SLES_DRIVE_DONE;
+  }
+  if (stop != 0.0) {
+    st[DRV_STATUS] = stop;
+    st[DRV_DT_APPLY] = NAN;
+    return;
+  }
+  st[DRV_T] = t + dt;
+  st[DRV_STEPS] += 1.0;
+  st[DRV_STATUS] = SLES_DRIVE_RUNNING;
+  st[DRV_DT_APPLY] = dt;
+}
+
 // ---------------------------------------------------------------------------
 // Optional per-kernel timing (measurement only).
 // ---------------------------------------------------------------------------
@@ -1048,7 +1130,8 @@ struct Fused {
 
 template <typename T, int N, class PS, class PX, class PZ>
 int step_impl(const sles_plan_s* p, int scheme, const sles_step_params* prm, void* uh_, void* uh0_, void* uh1_,
-              void* aux_, void* u_out, void* curl_out, double* scalars, void* work, hipStream_t s, void* stages) {
+              void* aux_, void* u_out, void* curl_out, double* scalars, void* work, hipStream_t s, void* stages,
+              const DriveArgs* drive) {
   using F = Fused<T, N, PS, PX, PZ>;
   cpx<T>* uh = (cpx<T>*)uh_;
   cpx<T>* uh0 = (cpx<T>*)uh0_;
@@ -1059,10 +1142,10 @@ int step_impl(const sles_plan_s* p, int scheme, const sles_step_params* prm, voi
   const int les = prm->les_model;
   const int nw = les ? 9 : 6;
   double* part = (double*)(W + nw * C);
-  const bool ddt = prm->dt_device != 0;
+  const bool ddt = prm->dt_device != 0 || drive;
   StepArgs a{};
   a.dt = prm->dt;
-  a.dt_dev = ddt ? scalars + 1 : nullptr;
+  a.dt_dev = drive ? drive->state + DRV_DT_APPLY : ddt ? scalars + 1 : nullptr;
   a.nu = prm->nu;
   a.forcing = prm->forcing;
   a.stages = scheme == SLES_SCHEME_RK4 ? stages : nullptr;
@@ -1096,8 +1179,12 @@ int step_impl(const sles_plan_s* p, int scheme, const sles_step_params* prm, voi
       constexpr int CHUNK = (F::ZB + NB - 1) / NB;
       double* part2 = part + F::ZB;
       hipLaunchKernelGGL(k_nanmax_chunks, dim3(NB), dim3(256), 0, s, (const double*)part, F::ZB, CHUNK, part2);
-      hipLaunchKernelGGL(k_cfl, dim3(1), dim3(64), 0, s, (const double*)part2, NB, prm->cfl, prm->dx, prm->nu,
-                         scalars);
+      if (drive)
+        hipLaunchKernelGGL(k_cfl_drive, dim3(1), dim3(64), 0, s, (const double*)part2, NB, prm->cfl, drive->cfl_first,
+                           prm->dx, prm->nu, drive->state);
+      else
+        hipLaunchKernelGGL(k_cfl, dim3(1), dim3(64), 0, s, (const double*)part2, NB, prm->cfl, prm->dx, prm->nu,
+                           scalars);
       if (check_launch("cfl")) return -1;
     }
     if (F::finish(p, les, uh, uh0, uh1, aux, W, a, s, fuse && st < 3, fuse && st == 3 && (u_out || curl_out)))
@@ -1121,14 +1208,16 @@ struct HasP4 {
 
 template <typename T, int N>
 int step_t(const sles_plan_s* p, int scheme, const sles_step_params* prm, void* uh, void* uh0, void* uh1, void* aux,
-           void* u_out, void* curl_out, double* scalars, void* work, hipStream_t s, void* stages) {
+           void* u_out, void* curl_out, double* scalars, void* work, hipStream_t s, void* stages,
+           const DriveArgs* drive) {
   using D = LinePlan<N>;
   if constexpr (HasP4<N>::value) {
     using Q = LinePlan4<N>;
     const int sel = (p->tune_strided ? 1 : 0) | (p->tune_xfwd ? 2 : 0) | (p->tune_z ? 4 : 0);
 #define SLES_SEL(K, A, B, Cz) \
   case K:                     \
-    return step_impl<T, N, A, B, Cz>(p, scheme, prm, uh, uh0, uh1, aux, u_out, curl_out, scalars, work, s, stages);
+    return step_impl<T, N, A, B, Cz>(p, scheme, prm, uh, uh0, uh1, aux, u_out, curl_out, scalars, work, s, stages, \
+                                     drive);
     switch (sel) {
       SLES_SEL(0, D, D, D)
       SLES_SEL(1, Q, D, D)
@@ -1141,13 +1230,16 @@ int step_t(const sles_plan_s* p, int scheme, const sles_step_params* prm, void* 
     }
 #undef SLES_SEL
   }
-  return step_impl<T, N, D, D, D>(p, scheme, prm, uh, uh0, uh1, aux, u_out, curl_out, scalars, work, s, stages);
+  return step_impl<T, N, D, D, D>(p, scheme, prm, uh, uh0, uh1, aux, u_out, curl_out, scalars, work, s, stages,
+                                  drive);
 }
 
 // Per-precision instantiation (sles_rhs_f64.hip / sles_rhs_f32.hip).
 int step_f64(const sles_plan_s* p, int scheme, const sles_step_params* prm, void* uh, void* uh0, void* uh1,
-             void* aux, void* u_out, void* curl_out, double* scalars, void* work, hipStream_t s, void* stages);
+             void* aux, void* u_out, void* curl_out, double* scalars, void* work, hipStream_t s, void* stages,
+             const DriveArgs* drive = nullptr);
 int step_f32(const sles_plan_s* p, int scheme, const sles_step_params* prm, void* uh, void* uh0, void* uh1,
-             void* aux, void* u_out, void* curl_out, double* scalars, void* work, hipStream_t s, void* stages);
+             void* aux, void* u_out, void* curl_out, double* scalars, void* work, hipStream_t s, void* stages,
+             const DriveArgs* drive = nullptr);
 
 }  // namespace sles

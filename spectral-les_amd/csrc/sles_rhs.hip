@@ -102,6 +102,31 @@ int sles_step_noalloc(sles_plan p, int scheme, const sles_step_params* prm, void
   return rc;
 }
 
+int sles_step_drive(sles_plan p, int scheme, const sles_step_params* prm, double cfl_first, int nsteps, void* u_hat,
+                    void* u_hat0, void* u_hat1, void* aux, void* u_out, void* curl_out, double* state, void* work,
+                    void* stream) {
+  if (!p || !prm || !u_hat || !state || !work) return fail("sles_step_drive: null argument");
+  if (scheme != SLES_SCHEME_RK4 && scheme != SLES_SCHEME_RK3 && scheme != SLES_SCHEME_ETD2)
+    return fail("sles_step_drive: scheme must be RK4, RK3 or ETD2");
+  if (prm->les_model < 0 || prm->les_model > 2) return fail("sles_step_drive: les_model must be 0, 1 or 2");
+  if (scheme != SLES_SCHEME_ETD2 && !u_hat0) return fail("sles_step_drive: RK schemes need u_hat0");
+  if (scheme == SLES_SCHEME_RK4 && !u_hat1) return fail("sles_step_drive: RK4 needs u_hat1");
+  if (scheme == SLES_SCHEME_ETD2 && !aux) return fail("sles_step_drive: ETD2 needs aux");
+  if (nsteps < 0) return fail("sles_step_drive: nsteps must be >= 0");
+  if (!sles_supported_n(p->n)) return fail("sles_step_drive: unsupported grid size");
+  hipStream_t s = (hipStream_t)stream;
+  const DriveArgs drv{state, cfl_first};
+  for (int i = 0; i < nsteps; ++i) {
+    const int rc = p->precision == SLES_FP64
+                       ? step_f64(p, scheme, prm, u_hat, u_hat0, u_hat1, aux, u_out, curl_out, nullptr, work, s,
+                                  nullptr, &drv)
+                       : step_f32(p, scheme, prm, u_hat, u_hat0, u_hat1, aux, u_out, curl_out, nullptr, work, s,
+                                  nullptr, &drv);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
 int sles_rk4_stage_states(sles_plan p, const sles_step_params* prm, void* u_hat, void* u_hat0, void* u_hat1,
                           void* stages, void* work, void* stream) {
   if (!p || !prm || !u_hat || !u_hat0 || !u_hat1 || !stages || !work)

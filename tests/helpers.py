@@ -46,3 +46,54 @@ def twins(N, seed=0, precision="fp64", k_cut=None, fused=False, **kw):
     g.U_hat1.copy_(g.U_hat)
     o.U_hat1[:] = o.U_hat
     return g, o, u
+
+
+def oracle_loop(o, file_out_time=0.0, No_Ek_plot=5, max_steps=None):
+    """The loop of spectralMethodSolver (src/SM_solver.py:18-68) on an oracle
+    domain: first dt with CFL 0.3 (:21-23), t = dt (:25), outputs at the loop
+    top (:40-45), stop at t >= T or dt < 1e-8 (:46-47), t += dt (:51), step,
+    dt from the CFL rule (:60-63), NaN break (:64-67).  Returns (counter1,
+    events, dts) with events a list of ("snapshot" | "Ek", t, data) in the
+    order the loop writes them (snapshot data: copies of U; Ek data:
+    (temp_k, temp_Ek) of calculate_Ek) and dts the dt of every step taken."""
+    d = o
+    d.dt = O.compute_dt(d.U, O.find_max_velocity(d.U), d.nu, dx=d.dx, CFL=0.3)
+    t = d.dt
+    file_time = 0.0
+    Ek_interval = d.T / No_Ek_plot if No_Ek_plot != 0 else 0
+    Ek_counter = 0
+    d.U_hat1[:] = d.U_hat
+    E = O.EkBins(d) if Ek_interval > 0 else None
+    events, dts = [], []
+    counter1 = 0
+    while True:
+        if file_out_time > 0 and t >= file_time:
+            events.append(("snapshot", t, np.array(d.U).copy()))
+            file_time += file_out_time
+        if Ek_interval > 0 and t >= Ek_counter:
+            O.calculate_Ek(d, E)
+            events.append(("Ek", t, (E.temp_k.copy(), E.temp_Ek.copy())))
+            Ek_counter += Ek_interval
+        if (t >= d.T) or (d.dt < 1e-8):
+            break
+        if max_steps is not None and counter1 >= max_steps:
+            break
+        t += d.dt
+        dts.append(d.dt)
+        O.step(d)
+        d.dt = O.compute_dt(d.U, O.find_max_velocity(d.U), d.nu, dx=d.dx, CFL=d.CFL)
+        if d.dt != d.dt:
+            break
+        counter1 += 1
+    return counter1, events, dts
+
+
+def rank1_shear_flow(N):
+    """u = m sin(n . x), n = (1, 2, 3), m = (1, 0.7, -0.8) (m . n = 0):
+    solenoidal, with the rank-1 gradient a_ij = n_i m_j cos(n . x) at every
+    point, whose B_beta is 0 in exact arithmetic and round-off of either sign
+    in floating point (m with integer components would make every product
+    exact and B exactly 0)."""
+    x, y, z = O.grid(N)
+    s = np.sin(x + 2 * y + 3 * z)
+    return s, 0.7 * s, -0.8 * s

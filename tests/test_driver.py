@@ -41,44 +41,147 @@ def test_get_index_partition():
 gpu = pytest.mark.gpu
 
 
-@gpu
-def test_driver_loop_matches_oracle_loop(tmp_path):
-    """spectralMethodSolver: dt from the CFL rule each step (first CFL 0.3)."""
-    if not torch.cuda.is_available():
-        pytest.skip("no GPU")
-    from helpers import random_field, rel
+def _read_csv(path):
+    import csv
+
+    with open(path, newline="") as f:
+        rows = list(csv.reader(f))
+    header = rows[0]
+    cols = {h: np.array([float(r[j]) for r in rows[1:] if r[j] != ""]) for j, h in enumerate(header)}
+    return header, cols
+
+
+def _driver_case(tmp_path, method, chunk, graph, N=16, final_time=1.0, file_out_time=0.0, No_Ek_plot=2,
+                 les="Smagorinsky", precision="fp64", CFL=0.5):
+    """(GPU domain after spectralMethodSolver, its return value, oracle domain
+    after the oracle loop, oracle (counter1, events, dts), output dir)."""
+    from helpers import oracle_loop, random_field
     from sles.initial import initial_velocity
     from sles.initialize_domain import InitializeDomain
     from sles.SM_solver import spectralMethodSolver
 
-    N = 16
     u = random_field(N, 31, k_cut=3.0)
-    kw = dict(nu=0.01, final_time=0.05, method="RK4", LES_ON=1, LES_model="Smagorinsky", Cs=0.17, CFL=0.5)
-    g = InitializeDomain(N=N, **kw)
+    kw = dict(nu=0.01, final_time=final_time, method=method, CFL=CFL)
+    if les:
+        kw.update(LES_ON=1, LES_model=les, Cs=0.17)
+    g = InitializeDomain(N=N, precision=precision, **kw)
     initial_velocity(g, *u)
+    out = tmp_path / f"{method}-{chunk}-{int(graph)}"
+    out.mkdir()
+    cwd = os.getcwd()
+    os.chdir(out)
+    try:
+        steps = spectralMethodSolver(g, file_out_time, No_Ek_plot=No_Ek_plot, chunk=chunk, graph=graph)
+    finally:
+        os.chdir(cwd)
+    o = O.Domain(N=N, **kw)
+    O.initial_velocity(o, *u)
+    ref = oracle_loop(o, file_out_time, No_Ek_plot)
+    return g, steps, o, ref, out
+
+
+@gpu
+@pytest.mark.parametrize("method,chunk,graph,nek,cfl", [("RK4", 1, False, 2, 0.5), ("RK4", 4, False, 3, 0.5),
+                                                        ("RK4", 10, True, 1, 0.5), ("RK3", 3, False, 2, 0.3),
+                                                        ("ETD2", 4, True, 1, 0.2)])
+def test_driver_loop_matches_oracle_loop(tmp_path, method, chunk, graph, nek, cfl):
+    """spectralMethodSolver on the device (dt, t and the stop rules per step
+    on the GPU, chunks of steps, optionally replayed HIP graphs) against the
+    oracle restatement of the reference loop: the same number of steps, the
+    state to 1e-10, and the E(k) CSV written at the same loop tops (same
+    t labels, same columns, bins to 1e-10)."""
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from helpers import rel
+
+    g, steps, o, (n, events, dts), out = _driver_case(tmp_path, method, chunk, graph, No_Ek_plot=nek, CFL=cfl)
+    assert steps == n and n > 8
+    assert rel(g.U_hat, o.U_hat) < 1e-10
+    assert rel(torch.stack(g.U), np.array(o.U)) < 1e-10
+    assert math.isclose(g.dt, o.dt, rel_tol=1e-10)
+    header, cols = _read_csv(out / "Ek_plot_data.csv")
+    ek = [e for e in events if e[0] == "Ek"]
+    assert header == [h for _, t, _ in ek for h in (f"k-{t:g}", f"Ek-{t:g}")]
+    for _, t, (k, e) in ek:
+        assert np.allclose(cols[f"k-{t:g}"], k, rtol=1e-12, atol=0)
+        assert np.allclose(cols[f"Ek-{t:g}"], e, rtol=1e-10, atol=1e-300)
+    loop = g._drive
+    if graph:
+        assert loop.replays > 0
+    # one read of the device loop state per chunk (plus one per output pause)
+    assert loop.host_reads <= math.ceil(n / chunk) + 2 * len(ek) + 4
+
+
+@gpu
+def test_driver_snapshots_match_oracle_loop(tmp_path):
+    """file_out_time > 0: a raw-binary + XDMF snapshot of U at every loop top
+    where output_h5 would write one (src/SM_solver.py:40-42,280-302), with
+    the reference's file numbering t // file_out_time, holding U of that
+    state."""
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import xml.etree.ElementTree as ET
+
+    N = 16
+    g, steps, o, (n, events, dts), out = _driver_case(tmp_path, "RK4", 4, False, N=N, final_time=0.6,
+                                                      file_out_time=0.2, No_Ek_plot=0)
+    snaps = [e for e in events if e[0] == "snapshot"]
+    assert len(snaps) >= 3
+    files = sorted(os.listdir(out / "dumpfile"))
+    assert files == sorted({f"dump{t // 0.2:g}.{x}" for _, t, _ in snaps for x in ("bin", "xdmf")})
+    for _, t, U in snaps:
+        tag = f"{t // 0.2:g}"
+        got = np.fromfile(out / "dumpfile" / f"dump{tag}.bin", dtype="<f8").reshape(3, N, N, N)
+        assert np.linalg.norm(got - U) / np.linalg.norm(U) < 1e-10
+        root = ET.parse(out / "dumpfile" / f"dump{tag}.xdmf").getroot()
+        attrs = root.findall(".//Attribute")
+        assert [a.get("Name") for a in attrs] == ["u", "v", "w"]
+        seeks = [int(a.find("DataItem").get("Seek")) for a in attrs]
+        assert seeks == [0, 8 * N ** 3, 16 * N ** 3]
+        assert root.find(".//Topology").get("Dimensions") == f"{N} {N} {N}"
+        assert root.find(".//Time").get("Value") == str(round(t // 0.2))
+
+
+@gpu
+def test_driver_dt_too_small_raises():
+    """compute_dt's exit for dt < 1e-6 (src/SM_solver.py:273-276) ends the
+    device loop with DtTooSmall, before any step."""
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from helpers import random_field
+    from sles.initial import initial_velocity
+    from sles.initialize_domain import InitializeDomain
+    from sles.SM_solver import DtTooSmall, spectralMethodSolver
+
+    g = InitializeDomain(N=16, nu=0.01, final_time=1.0, method="RK4")
+    initial_velocity(g, *(1e7 * random_field(16, 3, k_cut=3.0)))
+    before = g.U_hat.clone()
+    with pytest.raises(DtTooSmall):
+        spectralMethodSolver(g, 0.0, No_Ek_plot=0)
+    assert torch.equal(g.U_hat, before)
+
+
+def test_stale_ek_csv_is_removed(tmp_path):
+    """Ek_plot_variables removes an Ek_plot_data.csv left by an earlier run
+    (src/SM_solver.py:114-116), so the new run's file holds its own columns
+    only."""
+    from sles.SM_solver import Ek_plot_variables
+
+    class D:
+        N = 8
+        kx = np.fft.fftfreq(8, 1.0 / 8)
+        kz = np.abs(kx[:5])
+        device = "cpu"
+
     cwd = os.getcwd()
     os.chdir(tmp_path)
     try:
-        steps = spectralMethodSolver(g, 0.0, No_Ek_plot=2)
-        assert os.path.exists("Ek_plot_data.csv")
-        header = open("Ek_plot_data.csv").readline().strip().split(",")
-        assert header[0].startswith("k-") and header[1].startswith("Ek-")
+        with open("Ek_plot_data.csv", "w") as f:
+            f.write("k-0.5,Ek-0.5\n1,2\n")
+        Ek_plot_variables(D())
+        assert not os.path.exists("Ek_plot_data.csv")
     finally:
         os.chdir(cwd)
-    # oracle loop with the same rule
-    o = O.Domain(N=N, **kw)
-    O.initial_velocity(o, *u)
-    o.dt = O.compute_dt(o.U, O.find_max_velocity(o.U), o.nu, dx=o.dx, CFL=0.3)
-    t = o.dt
-    o.U_hat1[:] = o.U_hat
-    n = 0
-    while not (t >= o.T or o.dt < 1e-8):
-        t += o.dt
-        O.rk4_step(o)
-        o.dt = O.compute_dt(o.U, O.find_max_velocity(o.U), o.nu, dx=o.dx, CFL=o.CFL)
-        n += 1
-    assert steps == n
-    assert rel(g.U_hat, o.U_hat) < 1e-10
 
 
 @gpu
