@@ -485,7 +485,9 @@ __global__ void __launch_bounds__(ZLines<PL>::THREADS)
         const T sb = sqrt(B);
         const T aa = b[0][0] + b[1][1] + b[2][2];
         const bool on = (aa > T(1e-4) && sb > aa / T(1e7));
-        q = on ? sb * sqrt(T(1) / aa) : T(0);
+        // as the forward (src/vreman.py:119-123): B < 0 (round-off on a
+        // near-rank-1 gradient) gives sqrt -> NaN and NaN * 0 stays NaN
+        q = sb * (on ? sqrt(T(1) / aa) : T(0));
         // d q / d a_mr = q ((a G)_mr / B - a_mr / aa),  G = tr(beta) I - beta
         const T iB = (on && B > T(0)) ? T(1) / B : T(0), ia = on ? T(1) / aa : T(0);
 #pragma unroll

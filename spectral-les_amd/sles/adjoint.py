@@ -257,10 +257,12 @@ class DifferentiableLES:
 class _Advance(torch.autograd.Function):
     @staticmethod
     def forward(ctx, U0, Cs, solver):
-        solver._set_cs(Cs.item() if torch.is_tensor(Cs) else Cs)
+        cs = float(Cs.item() if torch.is_tensor(Cs) else Cs)
+        solver._set_cs(cs)
         out, ckpt = solver.run_forward(U0.detach().to(solver.d.cdtype))
         ctx.solver = solver
         ctx.ckpt = ckpt
+        ctx.cs = cs  # this forward's coefficient: the solver may run other forwards before this backward
         ctx.cs_dtype = Cs.dtype if torch.is_tensor(Cs) else torch.float64
         ctx.cs_device = Cs.device if torch.is_tensor(Cs) else solver.d.device
         ctx.in_dtype = U0.dtype
@@ -269,8 +271,16 @@ class _Advance(torch.autograd.Function):
     @staticmethod
     def backward(ctx, gU):
         solver = ctx.solver
-        lam = solver.run_backward(ctx.ckpt, gU.detach().to(solver.d.cdtype).contiguous())
-        gCs = solver.cs_grad.clone().reshape(()).to(dtype=ctx.cs_dtype, device=ctx.cs_device)
+        # recompute and differentiate at the Cs of this graph's forward pass
+        # (finite-difference probes, line searches or a second loss may have
+        # run the solver at another Cs since), then restore the solver's Cs
+        prev = solver.d.Cs
+        solver._set_cs(ctx.cs)
+        try:
+            lam = solver.run_backward(ctx.ckpt, gU.detach().to(solver.d.cdtype).contiguous())
+            gCs = solver.cs_grad.clone().reshape(()).to(dtype=ctx.cs_dtype, device=ctx.cs_device)
+        finally:
+            solver._set_cs(prev)
         return lam.to(ctx.in_dtype), gCs, None
 
 

@@ -132,12 +132,30 @@ def precision_tgv(N=64, steps=200, nu=1 / 1600, dt=0.01):
 
 def spectral_cut(U_hat, N_from, N_to):
     """Keep |kx|, |ky| < N_to/2 and kz <= N_to/2 of an N_from^3 spectrum, scaled
-    by (N_to/N_from)^3 for the unscaled forward-transform convention."""
+    by (N_to/N_from)^3 for the unscaled forward-transform convention (the
+    512^3 -> 256^3 target of the differentiable configuration: x 1/8).  For a
+    field whose modes all satisfy |k_i| < N_to/2 this is exactly the transform
+    of the field sampled on the N_to^3 grid."""
     h = N_to // 2
     sel = torch.cat([torch.arange(0, h), torch.arange(N_from - h, N_from)]).to(U_hat.device)
     out = U_hat[:, sel][:, :, sel][..., : h + 1].clone()
     out[:, h] = 0  # kx = -N_to/2 (|kx| < N_to/2 kept)
     out[:, :, h] = 0
+    return out * (float(N_to) / float(N_from)) ** 3
+
+
+def spectral_pad(U_hat, N_from, N_to):
+    """Zero-padded N_to^3 spectrum of an N_from^3 one (N_to > N_from), scaled
+    by (N_to/N_from)^3: the same band-limited field on the finer grid, with
+    the N_from/2 Nyquist planes dropped.  spectral_cut(spectral_pad(U)) == U
+    for a U without Nyquist modes."""
+    h = N_from // 2
+    out = torch.zeros(U_hat.shape[:-3] + (N_to, N_to, N_to // 2 + 1), dtype=U_hat.dtype, device=U_hat.device)
+    lo = torch.arange(0, h, device=U_hat.device)
+    src = torch.cat([lo, torch.arange(N_from - h + 1, N_from, device=U_hat.device)])
+    dst = torch.cat([lo, torch.arange(N_to - h + 1, N_to, device=U_hat.device)])
+    sub = U_hat[..., src, :, :][..., :, src, :][..., :h]
+    out[..., dst[:, None], dst[None, :], :h] = sub
     return out * (float(N_to) / float(N_from)) ** 3
 
 

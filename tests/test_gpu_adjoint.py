@@ -325,3 +325,120 @@ def test_keep_last_segment_is_bitwise_neutral(scheme):
     assert torch.equal(res[0][0], res[1][0])
     assert torch.equal(res[0][1], res[1][1])
     assert res[0][2] == res[1][2]
+
+
+def _torch_reference_etd2_grads(U0, Cs, N, nu, dt, nsteps, les, target):
+    """FP64 autograd on CPU through torch_reference.etd2_run (RK4 bootstrap,
+    then Cox-Matthews ETD2 with the SGS term in F)."""
+    U = U0.detach().cpu().to(torch.complex128).clone().requires_grad_(True)
+    c = torch.tensor(Cs, dtype=torch.float64, requires_grad=True)
+    X = TR.etd2_run(U, c, N, nu, dt, nsteps, les)
+    E = TR.shell_spectrum(X, N, kmax=target.numel())
+    loss = ((torch.log(E) - torch.log(target)) ** 2).sum()
+    loss.backward()
+    return float(loss.detach()), U.grad, float(c.grad)
+
+
+# Tolerances of the ETD2 + Vreman adjoint (the differentiable configuration:
+# Vreman c = 2.5 Cs^2 = 0.08, ETD2, checkpoints with recomputation) against
+# the FP64 autograd reference and central finite differences (h = 1e-3 in Cs;
+# in U0 a 1e-6 relative perturbation along a random direction in FP64, 1e-3
+# along U0 itself in FP32).  FP64: the adjoint is the exact transpose of the FP64 step, so it
+# meets autograd to 1e-8 and FD to its truncation error (1e-4).  FP32: the
+# forward and adjoint run in FP32 with FP64 sums; both gradients are compared
+# with the FP64 autograd reference to 2e-4 (FP32 round-off accumulated over
+# the window) and with FD of the FP32 loss to 2e-3 (FD of an FP32 loss:
+# rounding noise of ~1e-7 L over 2h).
+ETD2_TOL = {
+    "fp64": dict(loss=1e-10, grad_ref=1e-8, cs_ref=1e-8, cs_fd=1e-4, dir_fd=1e-5, eps=1e-6),
+    "fp32": dict(loss=1e-5, grad_ref=2e-4, cs_ref=2e-4, cs_fd=2e-3, dir_fd=2e-3, eps=1e-3),
+}
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("N", [16, 24])
+def test_etd2_vreman_adjoint_vs_autograd_and_fd(precision, N):
+    """ETD2 + Vreman (the differentiable LES configuration at small N): 7
+    steps with a checkpoint every 3 (three segments: two recomputed, the last
+    kept), loss = squared log-mismatch of E(k) against a target spectrum;
+    loss, dL/dU0 and dL/dCs against the FP64 torch-autograd reference and
+    central finite differences."""
+    from sles.adjoint import spectrum_loss
+
+    tol = ETD2_TOL[precision]
+    nu, dt, nsteps, kmax = 0.01, 0.005, 7, 6
+    Cs = math.sqrt(0.08 / 2.5)
+    U0 = _state(N, 51)
+    target = TR.shell_spectrum(torch.as_tensor(_state(N, 52).cpu()), N, kmax=kmax)
+    ref_loss, ref_gU, ref_gCs = _torch_reference_etd2_grads(U0, Cs, N, nu, dt, nsteps, "Vreman", target)
+
+    s = _solver(N, "Vreman", scheme="ETD2", nsteps=nsteps, dt=dt, nu=nu, precision=precision, K=3)
+    cd = s.d.cdtype
+    U0c = U0.to(cd)
+    U = U0c.clone().requires_grad_(True)
+    c = torch.tensor(Cs, dtype=torch.float64, device="cuda", requires_grad=True)
+    loss = spectrum_loss(s(U, c), target.cuda(), N, kmax=kmax)
+    loss.backward()
+    gU = U.grad.detach().cpu().to(torch.complex128)
+    gCs = float(c.grad)
+    assert math.isclose(float(loss.detach()), ref_loss, rel_tol=tol["loss"])
+    assert float((gU - ref_gU).abs().max()) < tol["grad_ref"] * float(ref_gU.abs().max())
+    assert math.isclose(gCs, ref_gCs, rel_tol=tol["cs_ref"]), (gCs, ref_gCs)
+
+    def L(U_, cv):
+        with torch.no_grad():
+            return float(spectrum_loss(s(U_, torch.tensor(cv, dtype=torch.float64)), target.cuda(), N, kmax=kmax))
+
+    h = 1e-3
+    fd = (L(U0c, Cs + h) - L(U0c, Cs - h)) / (2 * h)
+    assert abs(gCs - fd) < tol["cs_fd"] * abs(fd), (gCs, fd)
+    if precision == "fp64":
+        g = torch.Generator(device="cuda").manual_seed(53)
+        v = torch.complex(torch.randn(U0.shape, generator=g, device="cuda", dtype=torch.float64),
+                          torch.randn(U0.shape, generator=g, device="cuda", dtype=torch.float64))
+        v *= float(U0.abs().max()) * tol["eps"] / float(v.abs().max())
+    else:
+        # along U0 itself (d/dalpha L((1 + alpha) U0)): a random direction's
+        # derivative is below the FP32 loss's rounding noise
+        v = tol["eps"] * U0
+    dd = (L(U0c + v.to(cd), Cs) - L(U0c - v.to(cd), Cs)) / 2
+    assert abs(rdot(U.grad.to(torch.complex128), v) - dd) < tol["dir_fd"] * abs(dd), (rdot(U.grad, v), dd)
+
+
+@pytest.mark.parametrize("scheme", ["RK4", "ETD2"])
+def test_backward_uses_its_own_forward_cs(scheme):
+    """Two forwards through one solver at different Cs, then the backward of
+    the first: its gradients equal those of a forward + backward at that Cs
+    alone (the backward recomputes and differentiates at the Cs its own
+    forward ran with), and the solver's Cs is left as it was."""
+    from sles.adjoint import DifferentiableLES, spectrum_loss
+
+    N, nsteps, K = 16, 5, 2
+    U0 = _state(N, 61)
+    target = TR.shell_spectrum(torch.as_tensor(_state(N, 62).cpu()), N, kmax=6).cuda()
+
+    def make():
+        return DifferentiableLES(N=N, nu=0.01, dt=0.005, nsteps=nsteps, les_model="Vreman", scheme=scheme,
+                                 precision="fp64", checkpoint_every=K)
+
+    def graph(s, cs):
+        U = U0.clone().requires_grad_(True)
+        c = torch.tensor(cs, dtype=torch.float64, device="cuda", requires_grad=True)
+        return U, c, spectrum_loss(s(U, c), target, N, kmax=6)
+
+    s = make()
+    U1, c1, loss1 = graph(s, 0.10)
+    U2, c2, loss2 = graph(s, 0.25)  # a second forward at another Cs before the first backward
+    loss1.backward()
+    ref = make()
+    Ur, cr, lossr = graph(ref, 0.10)
+    lossr.backward()
+    assert torch.equal(U1.grad, Ur.grad)
+    assert float(c1.grad) == float(cr.grad)
+    assert s.d.Cs == 0.25
+    loss2.backward()
+    ref2 = make()
+    Ur2, cr2, lossr2 = graph(ref2, 0.25)
+    lossr2.backward()
+    assert torch.equal(U2.grad, Ur2.grad)
+    assert float(c2.grad) == float(cr2.grad)

@@ -181,3 +181,37 @@ def test_every_tuning_variant_matches_oracle(les, precision, N):
         assert {k: v for k, v in g.plan.get_tuning().items() if k in st} == st
         fused.rk4_step(g)
         assert rel(g.U_hat, o.U_hat) < tol, st
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_vreman_nan_semantics_on_rank1_flow(precision):
+    """The reference's Vreman NaN semantics (src/vreman.py:32,119-123) in both
+    precisions: on a flow whose gradient is rank 1 everywhere, B_beta = 0
+    exactly rounds to negative values at some points, sqrt gives NaN, and
+    NaN * 0 stays NaN -- in nu_R of the operator path and in the fused RHS,
+    as in the oracle (tests/test_oracle.py); Smagorinsky stays finite there."""
+    from helpers import rank1_shear_flow
+    from sles import fused
+    from sles.initial import initial_velocity
+    from sles.initialize_domain import InitializeDomain
+    from sles.vreman import Vreman_model
+
+    N = 16
+    u = rank1_shear_flow(N)
+    for model, expect_nan in (("Vreman", True), ("Smagorinsky", False)):
+        g = InitializeDomain(N=N, nu=0.01, method="RK4", precision=precision, **_kw(model))
+        initial_velocity(g, *u)
+        out = torch.empty_like(g.U_hat)
+        fused.rhs(g, out)
+        torch.cuda.synchronize()
+        assert bool(torch.isnan(torch.view_as_real(out)).any()) == expect_nan, model
+        if model == "Vreman":
+            Vreman_model(g, g.dU)
+            torch.cuda.synchronize()
+            nan = int(torch.isnan(g.nuR).sum())
+            print(f"[vreman nan] {precision}: {nan} of {N ** 3} points NaN", flush=True)
+            assert 0 < nan < N ** 3
+            if precision == "fp64":
+                # away from the NaN points nu_R is the switch's 0: sqrt(B) is FP64
+                # round-off (< 1e-6), below aa / 1e7 (~3e-6 for this flow)
+                assert bool((g.nuR[~torch.isnan(g.nuR)] == 0).all())

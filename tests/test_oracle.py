@@ -10,6 +10,7 @@ import math
 import numpy as np
 import pytest
 
+from helpers import rank1_shear_flow
 from oracle import sles_oracle as O
 
 
@@ -329,3 +330,114 @@ def test_calculate_Ek_taylor_green_modes():
     assert math.isclose(E.temp_Ek[0], expect0, rel_tol=1e-12)
     assert math.isclose(E.temp_k[0], np.mean(np.sqrt(in0)), rel_tol=1e-13)
     assert np.all(E.temp_Ek[1:] < 1e-20 * expect0)
+
+
+# ----------------------------------------------------------------------------
+# The LES branch of computeRHS (RK4_method.py:49-54,69-70): the pressure
+# projection acts on dU + T, and T is added once more after the viscous term.
+# ----------------------------------------------------------------------------
+def _les_domain(N, model, Cs=0.17, nu=0.01, forcing=0.0, seed=7):
+    d = O.Domain(N=N, nu=nu, forcing=forcing, LES_ON=1, LES_model=model, Cs=Cs, method="RK4")
+    d.U_hat[:] = random_solenoidal(d, seed)
+    for i in range(3):
+        d.U[i][:] = d.fft.backward(d.U_hat[i], d.U[i], normalize=True)
+    d.curl[:] = O.del_cross_U(d.fft, 3, d.U_hat, d.curl, d.K)
+    return d
+
+
+@pytest.mark.parametrize("model", ["Smagorinsky", "Vreman"])
+@pytest.mark.parametrize("forcing", [0.0, 0.2])
+def test_computeRHS_les_is_solenoidal(model, forcing):
+    """K . dU = 0 with LES on a solenoidal field: the projection of dU + T
+    (not of dU alone) makes the T added afterwards divergence-free.  T itself
+    is not solenoidal, so projecting only dU would fail this."""
+    d = _les_domain(12, model, forcing=forcing)
+    dU = np.zeros_like(d.U_hat)
+    O.computeRHS(d, dU, 1)
+    scale = np.max(np.abs(dU)) * np.max(np.abs(d.K))
+    assert np.max(np.abs(np.sum(d.K * dU, 0))) < 1e-12 * scale
+    assert np.max(np.abs(np.sum(d.K * d.temp_I, 0))) > 1e-3 * scale  # T has a gradient part
+
+
+@pytest.mark.parametrize("model", ["Smagorinsky", "Vreman"])
+def test_computeRHS_les_constant_nuR_closed_form(model, monkeypatch):
+    """With nu_R forced to a constant c through a hook, the SGS stress of a
+    solenoidal field is -c dealias K^2 U_hat, so computeRHS with LES equals
+    the no-LES RHS (pinned by the Taylor-Green closed form) minus
+    c dealias K^2 U_hat."""
+    c = 0.023
+    hook = "smagorinsky_nuR" if model == "Smagorinsky" else "vreman_nuR"
+
+    def const_nu(d):
+        d.nuR[:] = c
+
+    monkeypatch.setattr(O, hook, const_nu)
+    d = _les_domain(12, model, forcing=0.1)
+    dU = np.zeros_like(d.U_hat)
+    O.computeRHS(d, dU, 1)
+    plain = O.Domain(N=12, nu=d.nu, forcing=0.1, method="RK4")
+    plain.U_hat[:] = d.U_hat
+    ref = np.zeros_like(d.U_hat)
+    O.computeRHS(plain, ref, 1)
+    expect = ref - c * d.dealias * d.K2 * d.U_hat
+    assert np.max(np.abs(dU - expect)) < 1e-12 * np.max(np.abs(expect))
+
+
+@pytest.mark.parametrize("model", ["Smagorinsky", "Vreman"])
+def test_computeRHS_cs_zero_is_no_les(model):
+    """Cs = 0 gives nu_R = 0, T = 0 and the no-LES right-hand side."""
+    d = _les_domain(12, model, Cs=0.0)
+    dU = np.zeros_like(d.U_hat)
+    O.computeRHS(d, dU, 1)
+    plain = O.Domain(N=12, nu=d.nu, method="RK4")
+    plain.U_hat[:] = d.U_hat
+    ref = np.zeros_like(d.U_hat)
+    O.computeRHS(plain, ref, 1)
+    assert np.all(d.nuR == 0)
+    assert np.max(np.abs(dU - ref)) <= 1e-14 * np.max(np.abs(ref))
+
+
+def _rank1_negative_B_matrix():
+    """A rank-1 gradient a = x y^T (B_beta = 0 exactly) for which the
+    reference's difference-of-products formula (vreman.py:32-107, evaluated as
+    the oracle evaluates it) rounds to B_beta < 0."""
+    rng = np.random.default_rng(0)
+    for _ in range(1000):
+        x, y = rng.standard_normal(3), rng.standard_normal(3)
+        a = np.outer(x, y)
+        b = a.T @ a
+        B = (b[0, 0] * b[1, 1]) - (b[0, 1] * b[0, 1]) + (b[0, 0] * b[2, 2]) - (b[0, 2] * b[0, 2]) + (
+            b[1, 1] * b[2, 2]) - (b[1, 2] * b[1, 2])
+        if B < 0:
+            return a
+    raise AssertionError("no rank-1 matrix with B < 0 found")
+
+
+def test_vreman_nan_for_negative_B():
+    """vreman.py:32,119-123: sqrt of a negative B_beta (round-off on a rank-1
+    gradient) is NaN, the switch [aa > 1e-4 and sqrt(B) > aa/1e7] is then
+    false, and NaN * 0 stays NaN: nu_R is NaN at that point and finite
+    elsewhere."""
+    N = 4
+    d = O.Domain(N=N, LES_ON=1, LES_model="Vreman", Cs=0.17, method="RK4")
+    rng = np.random.default_rng(5)
+    A = rng.standard_normal((3, 3, N, N, N))
+    A[:, :, 1, 2, 3] = _rank1_negative_B_matrix()
+    _set_gradient(d, A)
+    O.vreman_nuR(d)
+    nan = np.isnan(d.nuR)
+    assert nan[1, 2, 3] and nan.sum() == 1
+
+
+def test_vreman_rhs_on_rank1_flow_is_nan_smagorinsky_finite():
+    """On the rank-1 flow the reference's Vreman RHS is NaN (some points round
+    to B < 0 and the NaN spreads through the transforms), while Smagorinsky's
+    (a square root of a sum of squares) stays finite."""
+    N = 16
+    for model, expect_nan in (("Vreman", True), ("Smagorinsky", False)):
+        d = O.Domain(N=N, nu=0.01, LES_ON=1, LES_model=model, Cs=0.17, method="RK4")
+        O.initial_velocity(d, *rank1_shear_flow(N))  # helpers.rank1_shear_flow
+        dU = np.zeros_like(d.U_hat)
+        O.computeRHS(d, dU, 0)
+        assert bool(np.isnan(d.nuR).any()) == expect_nan
+        assert bool(np.isnan(dU).any()) == expect_nan
