@@ -13,8 +13,11 @@ steps between two torch.cuda events on the stream the kernels run on.  The
 roofline object reports the dominant kernel's algorithmic bytes per launch
 over its mean launch time (hipEvents around each launch during K extra
 profiled steps).  The CPU baseline times the oracle (numpy/scipy FP64
-restatement of the reference, same FFT count and temporaries) for one RK4
-step of the same 256^3 LES on the host cores.
+restatement of the reference, same FFT count and temporaries) for one
+warm-up and three RK4 steps of the same 256^3 LES from the same seeded field
+on the host cores (all cores and one core).  The FP32 form's velocity rel-L2
+against the FP64 form after 100 steps (256^3 and 512^3) is reported beside
+its speed-up.
 """
 
 from __future__ import annotations
@@ -315,32 +318,78 @@ def roofline_of(res, N, precision, les=True):
     }, per_kernel
 
 
-def cpu_baseline(N, budget_s=60.0):
-    """One RK4 step of the oracle (FP64, scipy.fft with all host cores) on the
-    same 256^3 Smagorinsky workload."""
+def cpu_baseline(N, U_hat, steps=3):
+    """The oracle (numpy / scipy FP64 restatement of the reference, with its
+    108 FFTs per LES RK4 step and its numpy temporaries) on the same 256^3
+    Smagorinsky workload as the GPU: the same seeded initial spectrum U_hat
+    (complex128, host), dt from the CFL rule (0.3 for the first step, then
+    0.5).  Two variants, each one warm-up step then `steps` timed steps
+    (mean +- sd): all host cores (scipy.fft workers = cores, the reported
+    value) and one core (numpy pocketfft); the pointwise numpy work is
+    single-threaded in both."""
+    import statistics
+
     import numpy as np
 
     from oracle import sles_oracle as O
 
     cores = min(O.cpu_count(), 16)
-    d = O.Domain(N=N, nu=1e-4, method="RK4", LES_ON=1, LES_model="Smagorinsky", Cs=0.17, fft_workers=cores)
-    rng = np.random.default_rng(0)
-    u = rng.standard_normal((3, N, N, N))
-    O.initial_velocity(d, *u)
-    d.U_hat1[:] = d.U_hat
-    d.dt = 1e-3
-    t0 = time.perf_counter()
-    O.rk4_step(d)
-    t = time.perf_counter() - t0
+    cpu = platform.processor() or platform.machine()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    cpu = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    res = {}
+    for name, workers in (("all_cores", cores), ("one_core", None)):
+        d = O.Domain(N=N, nu=1e-4, method="RK4", LES_ON=1, LES_model="Smagorinsky", Cs=0.17, CFL=0.5,
+                     fft_workers=workers)
+        d.U_hat[:] = U_hat
+        for i in range(3):
+            d.U[i][:] = d.fft.backward(d.U_hat[i], d.U[i], normalize=True)
+        d.curl[:] = O.del_cross_U(d.fft, 3, d.U_hat, d.curl, d.K)
+        d.U_hat1[:] = d.U_hat
+        times = []
+        for k in range(steps + 1):
+            d.dt = O.compute_dt(d.U, O.find_max_velocity(d.U), d.nu, dx=d.dx, CFL=0.3 if k == 0 else 0.5)
+            t0 = time.perf_counter()
+            O.rk4_step(d)
+            times.append(time.perf_counter() - t0)
+        timed = times[1:]
+        m = statistics.mean(timed)
+        sd = statistics.stdev(timed) if len(timed) > 1 else 0.0
+        res[name] = {"steps_per_s": round(1.0 / m, 5), "s_per_step_mean": round(m, 3), "s_per_step_sd": round(sd, 3),
+                     "warmup_s": round(times[0], 3), "timed_steps": len(timed),
+                     "threads": cores if workers else 1}
+        log(f"[bench] cpu {name}: {m:.2f} +- {sd:.2f} s per step ({len(timed)} timed after 1 warm-up)")
+        del d
+    a = res["all_cores"]
     return {
-        "value": round(1.0 / t, 5),
+        "value": a["steps_per_s"],
         "unit": "RK4 steps/s",
         "cores": cores,
         "kind": "port",
-        "sample": f"1 RK4 step of the {N}^3 Smagorinsky LES (108 FFTs + numpy temporaries), "
-        f"oracle FP64 with scipy.fft workers={cores}, pointwise numpy single-threaded; {t:.1f} s",
-        "cpu": platform.processor() or platform.machine(),
+        "sample": f"1 warm-up + {steps} timed RK4 steps of the {N}^3 Smagorinsky LES from the GPU run's seeded "
+        f"initial field (CFL dt), oracle FP64 (108 FFTs + numpy temporaries per step) with scipy.fft "
+        f"workers={cores}, pointwise numpy single-threaded: {a['s_per_step_mean']} +- {a['s_per_step_sd']} s "
+        f"per step; one-core variant (numpy pocketfft): {res['one_core']['s_per_step_mean']} +- "
+        f"{res['one_core']['s_per_step_sd']} s per step",
+        "cpu": cpu,
+        "variants": res,
     }
+
+
+def fp32_error(N, steps=100):
+    """Velocity rel-L2 of the FP32 form against the FP64 form after `steps`
+    RK4 steps on the same seeded field and the FP64 run's dt sequence."""
+    from sles.studies import precision_study
+
+    r = precision_study(N=N, checkpoints=(steps,), seed=1)["results"][-1]
+    return {"steps": steps, "velocity_rel_l2": r["velocity_rel_l2"],
+            "max_rel_Ek_error_resolved_shells": r["max_rel_Ek_error_resolved_shells"]}
 
 
 def main(argv=None):
@@ -350,6 +399,7 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-steps", type=int, default=3, help="timed oracle steps per CPU-baseline variant")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-form comparison run")
     ap.add_argument("--no-512", action="store_true", help="skip the 512^3 FP64 / FP32 step timings in extra")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -401,6 +451,8 @@ def main(argv=None):
         extra["fp32_speedup_vs_fp64"] = round(v32 / value, 3)
         extra["kernels_fp32"] = roofline_of(r32, N, "fp32")[1]
         log(f"[bench] fp32 {v32:.2f} steps/s")
+        extra["fp32_vs_fp64_error"] = fp32_error(N)
+        log(f"[bench] fp32 vs fp64 after 100 steps: {extra['fp32_vs_fp64_error']}")
     if not a.no_512:
         for pr in ("fp64", "fp32"):
             log(f"[bench] 512^3 {pr}")
@@ -408,6 +460,11 @@ def main(argv=None):
             extra[f"grid512_{pr}"] = {"steps_per_s": round(1e4 / r["ms_total"], 3),
                                        "ms_per_step": round(r["ms_total"] / 10, 3), "steps": 10, "valid": r["ok"]}
             log(f"[bench] 512^3 {pr}: {extra[f'grid512_{pr}']}")
+        extra["grid512_fp32_speedup_vs_fp64"] = round(extra["grid512_fp32"]["steps_per_s"] /
+                                                      extra["grid512_fp64"]["steps_per_s"], 3)
+        if not a.no_fp32:
+            extra["grid512_fp32_vs_fp64_error"] = fp32_error(512)
+            log(f"[bench] 512^3 fp32 vs fp64 after 100 steps: {extra['grid512_fp32_vs_fp64_error']}")
     sizes = (N,) if a.no_512 else (N, 512)
     extra["fft_3d_batch3"] = {f"{n}_{pr}": fft_rates(n, pr) for n in sizes for pr in ("fp64", "fp32")}
     log(f"[bench] generic 3D FFT: {extra['fft_3d_batch3']}")
@@ -417,8 +474,13 @@ def main(argv=None):
     cpu = None
     if not a.no_cpu:
         extra["taylor_green_64_cpu_oracle"] = tgv_cpu()
-        log("[bench] CPU baseline (oracle, one RK4 step)")
-        cpu = cpu_baseline(N)
+        log("[bench] CPU baseline (oracle, 1 warm-up + 3 RK4 steps, all cores and one core)")
+        from sles.initial import isotropic_spectrum
+
+        Uh = isotropic_spectrum(N, k0=4.0, seed=1, urms=1.0).cpu().numpy()  # the GPU run's field (make_domain)
+        torch.cuda.empty_cache()
+        cpu = cpu_baseline(N, Uh, steps=a.cpu_steps)
+        del Uh
         log(f"[bench] cpu {cpu['value']} steps/s on {cpu['cores']} cores")
     line = {
         "metric": "RK4 steps/s at 256^3 (FP64 vs FP32 @ rel-L2<=1e-4); % HBM peak",
