@@ -2,8 +2,8 @@
 
 * Decaying isotropic LES 256^3 FP64 (Smagorinsky Cs = 0.17, nu = 1e-4, RK4,
   CFL rule: first dt 0.3, then 0.5): the device driver loop against the
-  oracle's loop for 10 steps -- velocity rel-L2 <= 1e-10, the dt sequence and
-  the calculate_Ek bins (src/SM_solver.py:154-192).
+  oracle's loop for 10 steps -- velocity rel-L2 <= 1e-10, the dt after the
+  last step and the calculate_Ek bins (src/SM_solver.py:154-192).
 * The same at 512^3 in the precision form: FP32 vs FP64, 3 RK4 steps on the
   FP64 run's dt sequence, velocity rel-L2 <= 1e-5.
 * Differentiable Vreman LES 256^3 FP32, ETD2, dt = 2e-3, checkpoints every 5
@@ -79,10 +79,16 @@ def test_256_fp64_driver_loop_matches_oracle():
     Eo = O.EkBins(o)
     O.calculate_Ek(o, Eo)
     assert np.allclose(E.temp_k, Eo.temp_k, rtol=1e-13, atol=0)
-    has = Eo.temp_Ek > 0
-    assert np.array_equal(E.temp_Ek > 0, has)
-    err = np.max(np.abs(E.temp_Ek[has] - Eo.temp_Ek[has]) / Eo.temp_Ek[has])
-    print(f"[256 fp64] calculate_Ek: {int(has.sum())} bins, max rel diff {err:.3e}", flush=True)
+    # every bin to 1e-13 of the peak bin; bins above 1e-8 of the peak (the
+    # rest hold round-off of modes ~1e-4 below the largest and smaller: the
+    # k^4 exp(-2 (k/4)^2) field has ~e^-200 of the peak at k = 40) to 1e-10
+    peak = float(Eo.temp_Ek.max())
+    absd = float(np.max(np.abs(E.temp_Ek - Eo.temp_Ek))) / peak
+    res = Eo.temp_Ek >= 1e-8 * peak
+    err = float(np.max(np.abs(E.temp_Ek[res] - Eo.temp_Ek[res]) / Eo.temp_Ek[res]))
+    print(f"[256 fp64] calculate_Ek: {len(Eo.temp_Ek)} bins, max |diff| / peak {absd:.3e}; {int(res.sum())} bins "
+          f"above 1e-8 of the peak, max rel diff {err:.3e}", flush=True)
+    assert absd < 1e-13
     assert err < 1e-10
 
 
