@@ -3,10 +3,15 @@
 
 Workload (one step = one iteration of the reference driver loop,
 src/SM_solver.py:39-68, without I/O): N = 256, Smagorinsky Cs = 0.17,
-Delta = dx = 2 pi/N, nu = 1e-4, RK4, dt from the CFL rule (0.5) evaluated on
-the device from max(u) of the state, and U / curl written back as the
-reference's step leaves them.  Initial field: seeded solenoidal random field
-with E(k) ~ k^4 exp(-2 (k/4)^2), u_rms = 1 (synthetic).
+Delta = dx = 2 pi/N, nu = 1e-4, RK4, dt from the CFL rule (0.3 for the first
+step, then 0.5) evaluated on the device from max(u) of the state.  The loop
+runs as the package's driver does (sles_step_drive): chunks of 10 steps with
+the loop state on the device, U / curl of the state written at the end of
+each chunk (the only point where the host can read them; each step
+recomputes the physical fields it needs from U_hat).  The same steps with
+one sles_step call each and U / curl written after every step are reported
+in extra.  Initial field: seeded solenoidal random field with
+E(k) ~ k^4 exp(-2 (k/4)^2), u_rms = 1 (synthetic).
 
 Prints one JSON line (rank 0, one GPU).  Timing: W warm-up steps, then K
 steps between two torch.cuda events on the stream the kernels run on.  The
@@ -112,44 +117,73 @@ def make_domain(N, precision, seed=1):
     return d
 
 
-def run_gpu(N, precision, steps, warmup, profile_steps=3):
+def run_gpu(N, precision, steps, warmup, profile_steps=3, chunk=10):
+    """The driver loop on the device (sles_step_drive: dt from the CFL rule
+    and the loop's stop rules evaluated per step on the GPU, first dt with
+    CFL 0.3, then 0.5), enqueued in chunks of `chunk` steps with U and curl
+    of the state written at the end of every chunk; `steps` timed steps
+    after `warmup` untimed ones.  Also times the same steps one sles_step
+    call each with U and curl written after every step."""
+    import math
+
     import torch
 
     from sles import _lib, fused
-    from sles.SM_solver import compute_dt, find_max_velocity
 
     d = make_domain(N, precision)
-    # first dt with CFL 0.3 (src/SM_solver.py:21-23), later ones on the device with CFL 0.5
-    umax = find_max_velocity(d.U)
-    d.dt = compute_dt(d.U, umax, d.nu, dx=d.dx, CFL=0.3)
-    fused.rk4_step(d, dt=d.dt)
-    for _ in range(warmup):
-        fused.rk4_step(d, dt_device=True, cfl=0.5)
+    state = torch.zeros(8, dtype=torch.float64, device=d.device)
+    state[2] = math.nan  # t starts at the first dt
+    state[6] = math.inf  # no output pauses
+    state[7] = math.inf  # no final time: the steps are counted here
+    prm = fused.params(d, cfl=0.5)
+    w = fused.work(d)
+
+    def drive(n):
+        _lib.call("sles_step_drive", d.plan.handle, fused.SCHEME_RK4, ctypes.byref(prm), 0.3, int(n),
+                  d.U_hat.data_ptr(), d.U_hat0.data_ptr(), d.U_hat1.data_ptr(), None, d._U.data_ptr(),
+                  d.curl.data_ptr(), state.data_ptr(), w.data_ptr(), d.stream)
+
+    def run(n):
+        while n > 0:
+            k = min(chunk, n)
+            drive(k)
+            n -= k
+
+    d.U_hat1.copy_(d.U_hat)
+    run(warmup)
     torch.cuda.synchronize()
     stream = torch.cuda.current_stream()
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
     e0.record(stream)
+    run(steps)
+    e1.record(stream)
+    e1.synchronize()
+    ms = e0.elapsed_time(e1)
+    st = state.cpu().numpy()
+    ok = bool(st[3] == 0 and st[4] == warmup + steps)
+    if not ok:
+        log(f"[bench] WARNING: driver state {st.tolist()}: steps were not all taken")
+    # the same steps, one sles_step call each, U and curl written after every step
+    e0.record(stream)
     for _ in range(steps):
         fused.rk4_step(d, dt_device=True, cfl=0.5)
     e1.record(stream)
     e1.synchronize()
-    ms = e0.elapsed_time(e1)
-    scal = d._scalars.cpu().numpy()
-    ok = bool(scal[1] >= 1e-6)
-    if not ok:
-        log(f"[bench] WARNING: CFL dt guard fired (dt = {scal[1]}, max u = {scal[0]}): steps were skipped")
-    # per-kernel timing over a few more steps
+    ms_each = e0.elapsed_time(e1)
+    # per-kernel timing over a few more steps (one chunk)
     ms_k = (ctypes.c_double * 8)()
     cnt_k = (ctypes.c_long * 8)()
-    _lib.call("sles_profile_enable", 1)
-    for _ in range(profile_steps):
-        fused.rk4_step(d, dt_device=True, cfl=0.5)
-    torch.cuda.synchronize()
-    _lib.call("sles_profile_read", ms_k, cnt_k, 7)
-    _lib.call("sles_profile_enable", 0)
-    kern = {KINDS[i]: (ms_k[i], cnt_k[i]) for i in range(7)}
-    out = dict(ms_total=ms, steps=steps, dt_last=float(scal[1]), umax_last=float(scal[0]), kernels=kern, ok=ok)
+    kern = {}
+    if profile_steps:
+        _lib.call("sles_profile_enable", 1)
+        drive(profile_steps)
+        torch.cuda.synchronize()
+        _lib.call("sles_profile_read", ms_k, cnt_k, 7)
+        _lib.call("sles_profile_enable", 0)
+        kern = {KINDS[i]: (ms_k[i], cnt_k[i]) for i in range(7)}
+    out = dict(ms_total=ms, ms_total_step_each=ms_each, steps=steps, dt_last=float(st[1]),
+               umax_last=float(st[0]), kernels=kern, ok=ok)
     del d
     torch.cuda.empty_cache()
     return out
@@ -248,8 +282,12 @@ def fft_rates(N, precision, reps=10):
         ("r2c_xcd", 1, lambda: _lib.call("sles_fft_r2c", plan.handle, u.data_ptr(), U.data_ptr(), 3, 1.0, s)),
         ("c2r_xcd", 1, lambda: _lib.call("sles_fft_c2r", plan.handle, U.data_ptr(), u.data_ptr(), U.data_ptr(),
                                          3, 1.0 / N ** 3, s)),
+        ("r2c_xcd_pipe", 3, lambda: _lib.call("sles_fft_r2c", plan.handle, u.data_ptr(), U.data_ptr(), 3, 1.0, s)),
+        ("c2r_xcd_pipe", 3, lambda: _lib.call("sles_fft_c2r", plan.handle, U.data_ptr(), u.data_ptr(), U.data_ptr(),
+                                              3, 1.0 / N ** 3, s)),
     ):
-        plan.set_tuning(xcd=xcd)
+        # xcd: bit 0 the XCD-aware tile order, bit 1 the pipelined wide tiles
+        plan.set_tuning(xcd=xcd & 1, fft_pipe=xcd >> 1)
         fn()
         torch.cuda.synchronize()
         e0 = torch.cuda.Event(enable_timing=True)
@@ -435,6 +473,8 @@ def main(argv=None):
     log(f"[bench] fp64 {value:.2f} steps/s ({ms_step:.3f} ms/step); dominant {roof['kernel']} "
         f"{roof['achieved']} GB/s")
     extra = {
+        "driver_chunk_steps": 10,
+        "steps_per_s_one_call_per_step": round(1e3 * a.steps / r64["ms_total_step_each"], 3),
         "grid_points_x_steps_per_s": round(value * N ** 3, 1),
         "step_roofline_frac": round(step_bytes * value / (HBM_PEAK_GBS * 1e9), 4),
         "step_algorithmic_GB": round(step_bytes / 1e9, 3),
@@ -447,6 +487,7 @@ def main(argv=None):
         r32 = run_gpu(N, "fp32", a.steps, a.warmup)
         v32 = 1e3 * a.steps / r32["ms_total"]
         extra["fp32_steps_per_s"] = round(v32, 3)
+        extra["fp32_steps_per_s_one_call_per_step"] = round(1e3 * a.steps / r32["ms_total_step_each"], 3)
         extra["fp32_valid"] = r32["ok"]
         extra["fp32_speedup_vs_fp64"] = round(v32 / value, 3)
         extra["kernels_fp32"] = roofline_of(r32, N, "fp32")[1]
@@ -458,7 +499,8 @@ def main(argv=None):
             log(f"[bench] 512^3 {pr}")
             r = run_gpu(512, pr, 10, 2, profile_steps=0)
             extra[f"grid512_{pr}"] = {"steps_per_s": round(1e4 / r["ms_total"], 3),
-                                       "ms_per_step": round(r["ms_total"] / 10, 3), "steps": 10, "valid": r["ok"]}
+                                       "ms_per_step": round(r["ms_total"] / 10, 3), "steps": 10, "valid": r["ok"],
+                                       "steps_per_s_one_call_per_step": round(1e4 / r["ms_total_step_each"], 3)}
             log(f"[bench] 512^3 {pr}: {extra[f'grid512_{pr}']}")
         extra["grid512_fp32_speedup_vs_fp64"] = round(extra["grid512_fp32"]["steps_per_s"] /
                                                       extra["grid512_fp64"]["steps_per_s"], 3)
@@ -496,8 +538,9 @@ def main(argv=None):
         "dtype": "f64",
         "data": "synthetic (seeded k^4 exp(-2(k/4)^2) solenoidal field, u_rms=1)",
         "config": {
-            "workload": f"decaying isotropic LES {N}^3, Smagorinsky Cs=0.17, nu=1e-4, RK4, CFL=0.5 (device dt), "
-            "U/curl written each step",
+            "workload": f"decaying isotropic LES {N}^3, Smagorinsky Cs=0.17, nu=1e-4, RK4, CFL 0.3 then 0.5, "
+            "driver loop on the device (dt and stop rules per step on the GPU, chunks of 10 steps, U/curl of the "
+            "state written at each chunk end)",
             "grid": N,
             "les_model": "Smagorinsky",
             "scheme": "RK4",

@@ -62,6 +62,9 @@ int sles_plan_info(sles_plan plan, int* n, int* nh, int* precision);
  *                             twice the columns (256- or 512-byte row runs, LDS tile
  *                             up to 128 KiB); N = 256 and 512 only (ignored
  *                             otherwise); values 0..3
+ *   SLES_TUNE_FFT_PIPE        generic transforms, wide tiles: one block runs every
+ *                             field of the batch, prefetching the next field (1),
+ *                             or one field per block (0, default)
  * The "radix-4" alternative plans: 4 points per thread (radix 4) for
  * N = 32, 64, 128, 256, 512; 6 points per thread (radix 3, then 2) for
  * N = 12, 24, 48, 96, 192, 384; ignored for N = 8, 16. */
@@ -71,7 +74,8 @@ enum {
   SLES_TUNE_Z_RADIX4 = 2,
   SLES_TUNE_XCD = 3,
   SLES_TUNE_FFT_WIDE = 4,
-  SLES_TUNE_FUSE_X = 5
+  SLES_TUNE_FUSE_X = 5,
+  SLES_TUNE_FFT_PIPE = 6
 };
 int sles_plan_set_tuning(sles_plan plan, int key, int value);
 int sles_plan_get_tuning(sles_plan plan, int key, int* value);
@@ -247,8 +251,11 @@ size_t sles_step_work_bytes(sles_plan plan, int les_model);
  * call leave U_hat unchanged.  The caller starts a run with t = NaN,
  * status = RUNNING, steps = 0 and sets T_PAUSE (+inf for none) and T_FINAL;
  * state holds SLES_DRIVE_STATE_SIZE doubles in device memory.  u_out /
- * curl_out receive U and curl of the state after every step (the state the
- * reference leaves), `work` as for sles_step.  No counterpart in the
+ * curl_out receive U and curl of the state at the end of the call (the
+ * state the reference leaves); they are written by the call's last step only,
+ * since no step reads them (each recomputes its physical fields from U_hat)
+ * and nothing else runs between the steps of a call.  `work` as for
+ * sles_step.  No counterpart in the
  * reference as an entry point (its loop is host Python). */
 enum {
   SLES_DRIVE_STATE_UMAX = 0,     /* max u of the last state examined */

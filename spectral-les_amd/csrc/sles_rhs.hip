@@ -117,11 +117,15 @@ int sles_step_drive(sles_plan p, int scheme, const sles_step_params* prm, double
   hipStream_t s = (hipStream_t)stream;
   const DriveArgs drv{state, cfl_first};
   for (int i = 0; i < nsteps; ++i) {
+    // U and curl are outputs only (each step recomputes the physical fields
+    // it needs from U_hat, and dt from the z pass's max u): the steps before
+    // the last would write values the next step overwrites unread
+    const bool last = i + 1 == nsteps;
+    void* uo = last ? u_out : nullptr;
+    void* co = last ? curl_out : nullptr;
     const int rc = p->precision == SLES_FP64
-                       ? step_f64(p, scheme, prm, u_hat, u_hat0, u_hat1, aux, u_out, curl_out, nullptr, work, s,
-                                  nullptr, &drv)
-                       : step_f32(p, scheme, prm, u_hat, u_hat0, u_hat1, aux, u_out, curl_out, nullptr, work, s,
-                                  nullptr, &drv);
+                       ? step_f64(p, scheme, prm, u_hat, u_hat0, u_hat1, aux, uo, co, nullptr, work, s, nullptr, &drv)
+                       : step_f32(p, scheme, prm, u_hat, u_hat0, u_hat1, aux, uo, co, nullptr, work, s, nullptr, &drv);
     if (rc) return rc;
   }
   return 0;
