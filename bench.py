@@ -282,12 +282,8 @@ def fft_rates(N, precision, reps=10):
         ("r2c_xcd", 1, lambda: _lib.call("sles_fft_r2c", plan.handle, u.data_ptr(), U.data_ptr(), 3, 1.0, s)),
         ("c2r_xcd", 1, lambda: _lib.call("sles_fft_c2r", plan.handle, U.data_ptr(), u.data_ptr(), U.data_ptr(),
                                          3, 1.0 / N ** 3, s)),
-        ("r2c_xcd_pipe", 3, lambda: _lib.call("sles_fft_r2c", plan.handle, u.data_ptr(), U.data_ptr(), 3, 1.0, s)),
-        ("c2r_xcd_pipe", 3, lambda: _lib.call("sles_fft_c2r", plan.handle, U.data_ptr(), u.data_ptr(), U.data_ptr(),
-                                              3, 1.0 / N ** 3, s)),
     ):
-        # xcd: bit 0 the XCD-aware tile order, bit 1 the pipelined wide tiles
-        plan.set_tuning(xcd=xcd & 1, fft_pipe=xcd >> 1)
+        plan.set_tuning(xcd=xcd)
         fn()
         torch.cuda.synchronize()
         e0 = torch.cuda.Event(enable_timing=True)

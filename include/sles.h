@@ -62,9 +62,6 @@ int sles_plan_info(sles_plan plan, int* n, int* nh, int* precision);
  *                             twice the columns (256- or 512-byte row runs, LDS tile
  *                             up to 128 KiB); N = 256 and 512 only (ignored
  *                             otherwise); values 0..3
- *   SLES_TUNE_FFT_PIPE        generic transforms, wide tiles: one block runs every
- *                             field of the batch, prefetching the next field (1),
- *                             or one field per block (0, default)
  * The "radix-4" alternative plans: 4 points per thread (radix 4) for
  * N = 32, 64, 128, 256, 512; 6 points per thread (radix 3, then 2) for
  * N = 12, 24, 48, 96, 192, 384; ignored for N = 8, 16. */
@@ -74,8 +71,7 @@ enum {
   SLES_TUNE_Z_RADIX4 = 2,
   SLES_TUNE_XCD = 3,
   SLES_TUNE_FFT_WIDE = 4,
-  SLES_TUNE_FUSE_X = 5,
-  SLES_TUNE_FFT_PIPE = 6
+  SLES_TUNE_FUSE_X = 5
 };
 int sles_plan_set_tuning(sles_plan plan, int key, int value);
 int sles_plan_get_tuning(sles_plan plan, int key, int* value);

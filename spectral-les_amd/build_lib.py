@@ -46,8 +46,17 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found (needed to build libsles.so)")
 
 
-def _compile(src: str) -> str:
+def _newest_header() -> float:
+    heads = [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".h")]
+    heads += [os.path.join(INCLUDE, f) for f in os.listdir(INCLUDE) if f.endswith(".h")]
+    return max(os.path.getmtime(h) for h in heads)
+
+
+def _compile(src: str, incremental: bool = False) -> str:
     obj = os.path.join(BUILD, os.path.basename(src).replace(".hip", ".o"))
+    if incremental and os.path.exists(obj):
+        if os.path.getmtime(obj) > max(os.path.getmtime(src), _newest_header()):
+            return obj
     cmd = [hipcc(), *HIPCC_FLAGS, "-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
@@ -55,13 +64,15 @@ def _compile(src: str) -> str:
     return obj
 
 
-def build(jobs: int | None = None, verbose: bool = True) -> str:
+def build(jobs: int | None = None, verbose: bool = True, incremental: bool = False) -> str:
+    """Compile every translation unit (incremental: only those older than
+    their source or any header) and link the library."""
     os.makedirs(BUILD, exist_ok=True)
     srcs = sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hip"))
     t0 = time.time()
     jobs = jobs or min(len(srcs), os.cpu_count() or 1, 8)
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
-        objs = list(ex.map(_compile, srcs))
+        objs = list(ex.map(lambda f: _compile(f, incremental), srcs))
     tmp = OUT + ".tmp"
     cmd = [hipcc(), "-shared", f"--offload-arch={ARCH}", "-o", tmp, *objs]
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -76,8 +87,9 @@ def build(jobs: int | None = None, verbose: bool = True) -> str:
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--jobs", type=int, default=None)
+    ap.add_argument("--incremental", action="store_true", help="recompile only out-of-date translation units")
     a = ap.parse_args(argv)
-    build(a.jobs)
+    build(a.jobs, incremental=a.incremental)
     return 0
 
 
