@@ -316,6 +316,22 @@ def tgv_cpu(N=64, steps=20):
             "sample": f"{steps} RK4 steps of the 64^3 Taylor-Green case, numpy pocketfft, {t:.2f} s"}
 
 
+def x_forward_pattern_ceiling(N, precision):
+    """The x-forward + update pass's own access pattern with no arithmetic
+    (sles_stream_probe, the radix-4 x-forward tile, XCD order): the mean over
+    the 4 RK4 stages' traffic mixes (9+9, 15+6, 15+6, 12+6 fields read +
+    written) of the algorithmic GB/s -- the bandwidth this tile shape and
+    traffic reach on this device, beside the 8 TB/s peak."""
+    from sles.studies import stream_ceiling
+
+    r = stream_ceiling(N, precision, mixes=((9, 9), (15, 6), (12, 6)), tiles=(0,), verbose=False)
+    R, C = field_bytes(N, precision)
+    ms = (r["tile0_9r9w"]["ms"] + 2 * r["tile0_15r6w"]["ms"] + r["tile0_12r6w"]["ms"]) / 4
+    nbytes = xfwd_mean_fields(True) * C
+    return {"GBps": round(nbytes / (ms * 1e-3) / 1e9, 1), "avg_launch_ms": round(ms, 4),
+            "kernel": "sles_stream_probe tile 0 (loads and stores of the x-forward tile, no arithmetic)"}
+
+
 def roofline_of(res, N, precision, les=True):
     R, C = field_bytes(N, precision)
     kern = res["kernels"]
@@ -465,6 +481,7 @@ def main(argv=None):
     ms_step = r64["ms_total"] / a.steps
     value = 1e3 / ms_step
     roof, per_kernel = roofline_of(r64, N, "fp64")
+    roof["pattern_ceiling"] = x_forward_pattern_ceiling(N, "fp64")
     step_bytes = step_bytes_minimal(N, "fp64", True)
     log(f"[bench] fp64 {value:.2f} steps/s ({ms_step:.3f} ms/step); dominant {roof['kernel']} "
         f"{roof['achieved']} GB/s")

@@ -359,6 +359,17 @@ int sles_axpby(sles_plan plan, int nfields, double a, const void* x, double b, v
 int sles_lincomb(sles_plan plan, int nfields, double a, const void* x, double b, const void* y, void* out,
                  void* stream);
 
+/* ---- measurement (no counterpart in the reference) ----
+ * The loads and stores of a strided x pass with no arithmetic: every thread
+ * of a tile of consecutive columns reads its rows of the nin spectral
+ * fields at `in` (contiguous) and writes their sum to the nout fields at
+ * `out`.  Its bandwidth is the ceiling of a pass with that tile shape:
+ * tile 0 the radix-4 x-forward tile (FP64 8 columns x 64 threads, 128-byte
+ * row runs; FP32 16 x 64), 1 16-column tiles (the radix-8 strided shape),
+ * 2 32-column tiles.
+ * N = 256 and 512 only; honours SLES_TUNE_XCD. */
+int sles_stream_probe(sles_plan plan, const void* in, int nin, void* out, int nout, int tile, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

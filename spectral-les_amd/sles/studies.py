@@ -343,6 +343,49 @@ def ensemble_adjoint(groups=((64, 8), (96, 8), (128, 8)), steps=200, checkpoint_
     return out
 
 
+def stream_ceiling(N=256, precision="fp64", mixes=((15, 6),), tiles=(0, 1, 2), xcd=1, reps=10, verbose=True):
+    """Bandwidth of the strided x-pass access pattern with no arithmetic
+    (sles_stream_probe): for each tile shape and (fields read, fields
+    written) mix, the algorithmic bytes over the mean launch time.  The
+    ceiling of a pass with that tile shape and traffic mix on this device."""
+    from . import _lib
+
+    prec = _lib.SLES_FP64 if precision == "fp64" else _lib.SLES_FP32
+    cdt = torch.complex128 if precision == "fp64" else torch.complex64
+    plan = _lib.Plan(N, prec, torch.device("cuda"))
+    plan.set_tuning(xcd=xcd)
+    nin = max(r for r, _ in mixes)
+    nout = max(w for _, w in mixes)
+    shape = (N, N, N // 2 + 1)
+    a = torch.zeros((nin,) + shape, dtype=cdt, device="cuda")
+    b = torch.zeros((max(nout, 1),) + shape, dtype=cdt, device="cuda")
+    C = a[0].numel() * a.element_size()
+    s = torch.cuda.current_stream().cuda_stream
+    out = {}
+    for tile in tiles:
+        for r, w in mixes:
+            def once():
+                _lib.call("sles_stream_probe", plan.handle, a.data_ptr(), r, b.data_ptr(), w, tile, s)
+
+            once()
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(reps):
+                once()
+            e1.record()
+            e1.synchronize()
+            ms = e0.elapsed_time(e1) / reps
+            gbs = (r + w) * C / (ms * 1e-3) / 1e9
+            out[f"tile{tile}_{r}r{w}w"] = {"ms": round(ms, 4), "GBps": round(gbs, 1)}
+            if verbose:
+                print(f"[stream] N={N} {precision} tile {tile} {r} read + {w} written: {ms:.4f} ms {gbs:.0f} GB/s",
+                      flush=True)
+    del a, b
+    torch.cuda.empty_cache()
+    return out
+
+
 def fft_throughput(sizes=(256, 512), precisions=("fp64", "fp32"), batch=3, reps=10):
     """Generic 3D transforms (sles_fft_r2c / sles_fft_c2r) of `batch` fields
     with each plan's default tiles: ms per call and algorithmic GB/s,
@@ -418,6 +461,9 @@ def main(argv=None):
     ff = sub.add_parser("fft")
     ff.add_argument("--n", type=int, nargs="+", default=[256, 512])
     ff.add_argument("--reps", type=int, default=10)
+    st = sub.add_parser("stream")
+    st.add_argument("--n", type=int, nargs="+", default=[256, 512])
+    st.add_argument("--precision", nargs="+", default=["fp64", "fp32"])
     ra = sub.add_parser("ensemble-adjoint")
     ra.add_argument("--steps", type=int, default=200)
     ra.add_argument("--out", default=None)
@@ -427,6 +473,13 @@ def main(argv=None):
         if a.out:
             with open(a.out, "w") as f:
                 json.dump(res, f, indent=1)
+        return
+    if a.cmd == "stream":
+        # mixes of the x passes: the x-forward + update of RK4 stages 0 / 1-2 / 3
+        # (9+9, 15+6, 12+6 fields), the x-inverse (3+6) and a plain 1+1
+        for n in a.n:
+            for prec in a.precision:
+                stream_ceiling(n, prec, mixes=((9, 9), (15, 6), (12, 6), (3, 6), (1, 1)))
         return
     if a.cmd == "fft":
         fft_throughput(tuple(a.n), reps=a.reps)
