@@ -1,12 +1,17 @@
-"""Autotuner for the fused time-step kernels.
+"""Autotuner for the fused time-step kernels and the generic transforms.
 
 For one (N, precision, LES model) it times every combination of the kernel
 variants a plan exposes (``sles_plan_set_tuning``: radix-8 or radix-4 line
-plans for the strided passes, the x-forward/update pass and the z kernel, and
-the XCD-aware tile order) on a few RK4 steps of a synthetic field, and keeps
-the fastest.  Results persist in a JSON table (default: ``tuning.json`` next
-to this file, override with ``SLES_TUNING_TABLE``) that
-``InitializeDomain`` applies to its plan.
+plans for the strided passes, the x-forward/update pass and the z kernel --
+which also fixes each pass's tile: the columns per block follow from the
+plan's threads per line and the 64 KiB LDS tile -- the XCD-aware tile order
+and the fusion of the x-forward with the next x-inverse) on a few RK4 steps
+of a synthetic field, and keeps the fastest.  For N = 256 and 512 it also
+times the four tile widths of the generic transforms' strided passes
+(``fft_wide``: twice the columns per tile in the x and/or y pass) on a
+batched forward + backward transform.  Results persist in a JSON table
+(default: ``tuning.json`` next to this file, override with
+``SLES_TUNING_TABLE``) that ``InitializeDomain`` applies to its plan.
 
     python -m sles.autotune --n 256 --precision fp64 --les smagorinsky
 
@@ -28,7 +33,8 @@ import torch
 from . import _lib
 
 TABLE = os.environ.get("SLES_TUNING_TABLE", os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning.json"))
-KNOBS = ("strided_radix4", "xfwd_radix4", "z_radix4", "xcd", "fuse_x")
+KNOBS = ("strided_radix4", "xfwd_radix4", "z_radix4", "xcd", "fuse_x", "fft_wide")
+WIDE_SIZES = (256, 512)  # sizes with wide generic-transform tiles
 DEFAULTS = dict(strided_radix4=0, xfwd_radix4=0, z_radix4=0, xcd=1, fuse_x=1)  # a new plan's settings
 RADIX4_SIZES = (12, 24, 32, 48, 64, 96, 128, 192, 256, 384, 512)  # sizes with an alternative line plan
 
@@ -50,7 +56,7 @@ def apply(plan, N, precision, les_model, path=TABLE):
     entry = load_table(path).get(key(N, precision, les_model))
     if entry is None:
         return None
-    plan.set_tuning(**{k: entry["settings"][k] for k in KNOBS if k in entry["settings"]})
+    plan.set_tuning(**{k: entry["settings"][k] for k in KNOBS if k in entry["settings"]})  # fft_wide: N 256, 512
     return entry["settings"]
 
 
@@ -65,6 +71,26 @@ def candidates(N, precision=None):
     fuse = (0, 1) if precision is None or has_fused_x(N, precision) else (1,)
     for s, x, z, c, f in itertools.product(radix, radix, radix, (0, 1), fuse):
         yield dict(strided_radix4=s, xfwd_radix4=x, z_radix4=z, xcd=c, fuse_x=f)
+
+
+def time_fft_wide(domain, wide, reps=5):
+    """Mean ms of a batched (3 fields) forward + backward generic transform
+    with fft_wide = `wide`."""
+    d = domain
+    d.plan.set_tuning(fft_wide=wide)
+    u = d._U
+    X = d.fft.forward(u).clone()
+    d.fft.backward(X)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        d.fft.forward(u)
+        d.fft.backward(X)
+    e1.record()
+    e1.synchronize()
+    return e0.elapsed_time(e1) / reps
 
 
 def time_variant(domain, settings, steps=3, warmup=1):
@@ -100,6 +126,14 @@ def tune(N, precision="fp64", les="smagorinsky", steps=3, path=TABLE, verbose=Tr
             print(f"[autotune] N={N} {precision} {les}: {st} -> {ms:.3f} ms/step", flush=True)
     results.sort(key=lambda r: r[0])
     best_ms, best = results[0]
+    best = dict(best)
+    if N in WIDE_SIZES:
+        d.plan.set_tuning(**best)
+        wide = [(time_fft_wide(d, w), w) for w in range(4)]
+        if verbose:
+            for ms, w in wide:
+                print(f"[autotune] N={N} {precision} generic transforms fft_wide={w} -> {ms:.3f} ms", flush=True)
+        best["fft_wide"] = min(wide)[1]
     table = load_table(path)
     table[key(N, precision, d.sgs_model)] = {
         "settings": best,
