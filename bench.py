@@ -298,6 +298,28 @@ def fft_rates(N, precision, reps=10):
     return out
 
 
+def ensemble_rates(steps=100, steps_per_graph=10):
+    """8 x 64^3 + 8 x 96^3 + 8 x 128^3 FP32 Smagorinsky RK4 members (own seeds,
+    Cs ~ U[0.10, 0.20], CFL dt on the device), graph replays of 10 ensemble
+    steps: aggregate grid points x steps / s with one launch per pass per grid
+    size (batched) and with one sles_step_drive call per member."""
+    import torch
+
+    from sles.studies import ensemble_rate
+
+    out = {"members": "8 x 64^3 + 8 x 96^3 + 8 x 128^3", "steps": steps, "steps_per_graph": steps_per_graph}
+    for batched in (True, False):
+        log(f"[bench] ensemble {'batched' if batched else 'one call per member'}")
+        r = ensemble_rate(steps=steps, steps_per_graph=steps_per_graph, batched=batched)
+        key = "batched" if batched else "per_member"
+        out[key] = {"grid_points_x_steps_per_s": round(r["grid_points_x_steps_per_s"], 1),
+                    "all_members_stepped": r["all_members_stepped"]}
+        torch.cuda.empty_cache()
+    out["batched_speedup"] = round(out["batched"]["grid_points_x_steps_per_s"] /
+                                   out["per_member"]["grid_points_x_steps_per_s"], 3)
+    return out
+
+
 def tgv_cpu(N=64, steps=20):
     """The same Taylor-Green case on the oracle (numpy FP64, one process)."""
     import numpy as np
@@ -452,6 +474,7 @@ def main(argv=None):
     ap.add_argument("--cpu-steps", type=int, default=3, help="timed oracle steps per CPU-baseline variant")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-form comparison run")
     ap.add_argument("--no-512", action="store_true", help="skip the 512^3 FP64 / FP32 step timings in extra")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the ensemble throughput in extra")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a plain 1 GiB device copy 5 times (reference kernel for HBM counters)")
     a = ap.parse_args(argv)
@@ -526,6 +549,9 @@ def main(argv=None):
     log("[bench] 64^3 Taylor-Green (no LES) FP64")
     extra["taylor_green_64_fp64"] = tgv_gpu()
     log(f"[bench] tgv64 {extra['taylor_green_64_fp64']['steps_per_s']} steps/s")
+    if not a.no_ensemble:
+        extra["ensemble_fp32"] = ensemble_rates()
+        log(f"[bench] ensemble: {extra['ensemble_fp32']}")
     cpu = None
     if not a.no_cpu:
         extra["taylor_green_64_cpu_oracle"] = tgv_cpu()

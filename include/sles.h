@@ -276,6 +276,26 @@ int sles_step_drive(sles_plan plan, int scheme, const sles_step_params* params, 
                     void* u_hat, void* u_hat0, void* u_hat1, void* aux, void* u_out, void* curl_out, double* state,
                     void* work, void* stream);
 
+/* sles_step_drive for `members` independent runs of one grid size (an
+ * ensemble of seeds and SGS coefficients), advanced by the same launches:
+ * every pass of a step is one launch over all members (blockIdx.y = member),
+ * so small grids fill the device together.  Each member is one run of
+ * spectralMethodSolver's loop (src/SM_solver.py:18-68) with its own state
+ * (own dt, t, status and stop rules, exactly as sles_step_drive); the
+ * members share the plan, the scheme and every field of `params` except
+ * sgs_coeff, which comes per member from the device array sgs_coeff
+ * (`members` doubles; may be NULL without LES).  Memory is packed by member:
+ * member m's u_hat (and u_hat0, u_hat1, aux) starts 3 C elements after member
+ * m-1's (C = N * N * (N/2+1)), its driver state SLES_DRIVE_STATE_SIZE doubles
+ * after; `work` holds sles_step_batch_work_bytes(plan, les_model, members)
+ * bytes.  No U / curl output (read a member's U_hat instead).  No counterpart
+ * in the reference, which runs one field per process. */
+#define SLES_BATCH_MAX_MEMBERS 65535
+size_t sles_step_batch_work_bytes(sles_plan plan, int les_model, int members);
+int sles_step_drive_batch(sles_plan plan, int scheme, const sles_step_params* params, int members,
+                          const double* sgs_coeff, double cfl_first, int nsteps, void* u_hat, void* u_hat0,
+                          void* u_hat1, void* aux, double* state, void* work, void* stream);
+
 /* sles_step without a workspace argument, for callers that cannot allocate
  * device memory (code under HIP graph capture, tight serving loops,
  * harnesses that forbid hipMalloc).  No counterpart in the reference.  The

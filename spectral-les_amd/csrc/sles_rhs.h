@@ -29,6 +29,7 @@
 
 #include <cmath>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "sles.h"
@@ -74,6 +75,29 @@ struct DriveArgs {
   double cfl_first;
 };
 
+// Batched launches (sles_step_drive_batch): blockIdx.y is the member, whose
+// U_hat (and U_hat0, U_hat1, aux), workspace (W and the CFL partials behind
+// it) and driver state lie at fixed byte strides from the first member's.
+// Single-member launches pass zero strides and a one-row grid.
+struct Members {
+  long state;           // bytes between members' U_hat (also U_hat0, U_hat1, aux)
+  long work;            // bytes between members' workspaces
+  long drive;           // bytes between members' driver states
+  const double* coeff;  // per-member SGS coefficient (device), or nullptr: the launch's own
+};
+
+// The members of a batched step: count grid rows at the strides of mb.
+struct Batch {
+  int count;
+  Members mb;
+};
+
+template <class P>
+__device__ __forceinline__ P* member_ptr(P* p, long stride) {
+  using B = typename std::conditional<std::is_const<P>::value, const char, char>::type;
+  return reinterpret_cast<P*>(reinterpret_cast<B*>(p) + (long)blockIdx.y * stride);
+}
+
 // ---------------------------------------------------------------------------
 // Tile geometry of the strided passes (N * (N/2+1) columns, TK per block).
 // ---------------------------------------------------------------------------
@@ -99,7 +123,10 @@ struct Cols {
 // ---------------------------------------------------------------------------
 template <typename T, int N, int LES, class PL>
 __global__ void __launch_bounds__((WideShape<T, N, PL>::THREADS))
-    k_xinv(const cpx<T>* __restrict__ uh, cpx<T>* __restrict__ W, long C, const cpx<T>* __restrict__ tw, int xcd) {
+    k_xinv(const cpx<T>* __restrict__ uh, cpx<T>* __restrict__ W, long C, const cpx<T>* __restrict__ tw, int xcd,
+           Members mb) {
+  uh = member_ptr(uh, mb.state);
+  W = member_ptr(W, mb.work);
   using S = WideShape<T, N, PL>;
   constexpr int P = S::P, NT = S::NT, TK = S::TK;
   constexpr int PPB = N * TK;  // ping-pong exchange buffers
@@ -167,7 +194,8 @@ __global__ void __launch_bounds__((WideShape<T, N, PL>::THREADS))
 // ---------------------------------------------------------------------------
 template <typename T, int N, int LES, class PL>
 __global__ void __launch_bounds__((WideShape<T, N, PL>::THREADS))
-    k_yinv(cpx<T>* W, long C, const cpx<T>* __restrict__ tw, int xcd) {
+    k_yinv(cpx<T>* W, long C, const cpx<T>* __restrict__ tw, int xcd, Members mb) {
+  W = member_ptr(W, mb.work);
   using S = WideShape<T, N, PL>;
   constexpr int P = S::P, NT = S::NT, TK = S::TK, NH = N / 2 + 1;
   constexpr int PPB = N * TK;  // ping-pong exchange buffers
@@ -362,7 +390,10 @@ __device__ __forceinline__ void block_max_store(double v, double* out) {
 
 template <typename T, int N, class PL, int MODEL>
 __global__ void __launch_bounds__(ZLines<PL>::THREADS)
-    k_zles(cpx<T>* W, long C, T scale, T coeff, double* umax_part, const cpx<T>* __restrict__ tw) {
+    k_zles(cpx<T>* W, long C, T scale, T coeff, double* umax_part, const cpx<T>* __restrict__ tw, Members mb) {
+  W = member_ptr(W, mb.work);
+  if (umax_part) umax_part = member_ptr(umax_part, mb.work);
+  if (mb.coeff) coeff = (T)mb.coeff[blockIdx.y];
   constexpr int P = PL::P, NT = PL::NT, NH = N / 2 + 1, KH = P / 2 + 1;
   constexpr int LPB = ZLines<PL>::LPB;
   constexpr bool WV = WaveLocal<PL>::value;  // each line inside one wavefront: no block barriers
@@ -490,7 +521,9 @@ __global__ void __launch_bounds__(ZLines<PL>::THREADS)
 
 template <typename T, int N, class PL>
 __global__ void __launch_bounds__(ZLines<PL>::THREADS)
-    k_zns(cpx<T>* W, long C, T scale, double* umax_part, const cpx<T>* __restrict__ tw) {
+    k_zns(cpx<T>* W, long C, T scale, double* umax_part, const cpx<T>* __restrict__ tw, Members mb) {
+  W = member_ptr(W, mb.work);
+  if (umax_part) umax_part = member_ptr(umax_part, mb.work);
   constexpr int P = PL::P, NT = PL::NT, NH = N / 2 + 1, KH = P / 2 + 1;
   constexpr int LPB = ZLines<PL>::LPB;
   constexpr bool WV = WaveLocal<PL>::value;  // each line inside one wavefront: no block barriers
@@ -558,7 +591,8 @@ __global__ void __launch_bounds__(ZLines<PL>::THREADS)
 // ---------------------------------------------------------------------------
 template <typename T, int N, int LES, class PL>
 __global__ void __launch_bounds__((WideShape<T, N, PL>::THREADS))
-    k_yfwd(cpx<T>* W, long C, const cpx<T>* __restrict__ tw, int xcd) {
+    k_yfwd(cpx<T>* W, long C, const cpx<T>* __restrict__ tw, int xcd, Members mb) {
+  W = member_ptr(W, mb.work);
   using S = WideShape<T, N, PL>;
   constexpr int P = S::P, NT = S::NT, TK = S::TK, NH = N / 2 + 1;
   // ping-pong exchange buffers in FP32 only: in FP64 they would double the
@@ -761,7 +795,13 @@ using XShape = StridedShape<T, N, PL, (sizeof(T) == 4 && PL::P == 4) ? 1024 : 51
 template <typename T, int N, int LES, class PL, bool FUSE = false, int FLES = LES>
 __global__ void __launch_bounds__((XShape<T, N, PL>::THREADS))
     k_xfwd(cpx<T>* __restrict__ W, long C, cpx<T>* uh, cpx<T>* uh0, cpx<T>* uh1, cpx<T>* aux, StepArgs a,
-           const cpx<T>* __restrict__ tw, int xcd) {
+           const cpx<T>* __restrict__ tw, int xcd, Members mb) {
+  W = member_ptr(W, mb.work);
+  uh = member_ptr(uh, mb.state);
+  if (uh0) uh0 = member_ptr(uh0, mb.state);
+  if (uh1) uh1 = member_ptr(uh1, mb.state);
+  if (aux) aux = member_ptr(aux, mb.state);
+  if (a.dt_dev) a.dt_dev = member_ptr(a.dt_dev, mb.drive);
   using S = XShape<T, N, PL>;
   constexpr int P = S::P, NT = S::NT, TK = S::TK, NH = N / 2 + 1;
   constexpr int PPB = N * TK;  // ping-pong exchange buffers
@@ -881,7 +921,9 @@ __global__ void __launch_bounds__((XShape<T, N, PL>::THREADS))
 
 // First level of the CFL reduction: block b folds part[b*chunk, (b+1)*chunk)
 // (NaN-propagating max) into out[b].
-static __global__ void k_nanmax_chunks(const double* part, int np, int chunk, double* out) {
+static __global__ void k_nanmax_chunks(const double* part, int np, int chunk, double* out, Members mb) {
+  part = member_ptr(part, mb.work);
+  out = member_ptr(out, mb.work);
   double m = -INFINITY;
   const int lo = blockIdx.x * chunk, hi = lo + chunk < np ? lo + chunk : np;
   for (int i = lo + threadIdx.x; i < hi; i += blockDim.x) m = (isnan(m) || m > part[i]) ? m : part[i];
@@ -932,7 +974,9 @@ static __global__ void k_cfl(const double* part, int np, double cfl, double dx, 
 // t += dt (:51).  state[DRV_DT_APPLY] receives the dt of the step, or NaN
 // when it is not taken (the x-forward kernels then leave U_hat unchanged).
 static __global__ void k_cfl_drive(const double* part, int np, double cfl, double cfl_first, double dx, double nu,
-                                   double* st) {
+                                   double* st, Members mb) {
+  part = member_ptr(part, mb.work);
+  st = member_ptr(st, mb.drive);
   double m = -INFINITY;
   for (int i = threadIdx.x; i < np; i += blockDim.x) m = (isnan(m) || m > part[i]) ? m : part[i];
   __shared__ double s[1024];
@@ -1039,35 +1083,35 @@ struct Fused {
   static constexpr bool FUSE_X = sizeof(T) == 4 || N <= 64;
 
   // skip_xinv: W0-5 already hold the x-inverse output (fused into the previous x-forward)
+  // nm members (grid rows) at the strides of mb
   static int stage(const sles_plan_s* p, int les, T coeff, cpx<T>* uh, cpx<T>* W, double* part, hipStream_t s,
-                   bool skip_xinv = false) {
+                   bool skip_xinv, const Members& mb, int nm) {
     const long C = ncplx(p);
     const cpx<T>* tw = (const cpx<T>*)p->tw;
     const int xcd = p->tune_xcd;
     constexpr long NCOL = (long)N * (N / 2 + 1);  // columns of every strided pass
-    const dim3 gs((unsigned)((NCOL + SS::TK - 1) / SS::TK)), bs(SS::THREADS);
+    const dim3 gs((unsigned)((NCOL + SS::TK - 1) / SS::TK), (unsigned)nm), bs(SS::THREADS);
+    const dim3 gz((unsigned)ZB, (unsigned)nm), bz(ZLines<PZ>::THREADS);
     const T scale = T(1.0 / ((double)N * N * N));
     if (les) {
       if (!skip_xinv) {
         ProfScope ps_(PK_XINV, s);
-        hipLaunchKernelGGL((k_xinv<T, N, 1, PS>), gs, bs, 0, s, uh, W, C, tw, xcd);
+        hipLaunchKernelGGL((k_xinv<T, N, 1, PS>), gs, bs, 0, s, uh, W, C, tw, xcd, mb);
       }
-      { ProfScope ps_(PK_YINV, s); hipLaunchKernelGGL((k_yinv<T, N, 1, PS>), gs, bs, 0, s, W, C, tw, xcd); }
+      { ProfScope ps_(PK_YINV, s); hipLaunchKernelGGL((k_yinv<T, N, 1, PS>), gs, bs, 0, s, W, C, tw, xcd, mb); }
       ProfScope ps_(PK_Z, s);
       if (les == SLES_LES_SMAGORINSKY)
-        hipLaunchKernelGGL((k_zles<T, N, PZ, SLES_LES_SMAGORINSKY>), dim3(ZB), dim3(ZLines<PZ>::THREADS), 0, s, W, C,
-                           scale, coeff, part, tw);
+        hipLaunchKernelGGL((k_zles<T, N, PZ, SLES_LES_SMAGORINSKY>), gz, bz, 0, s, W, C, scale, coeff, part, tw, mb);
       else
-        hipLaunchKernelGGL((k_zles<T, N, PZ, SLES_LES_VREMAN>), dim3(ZB), dim3(ZLines<PZ>::THREADS), 0, s, W, C,
-                           scale, coeff, part, tw);
+        hipLaunchKernelGGL((k_zles<T, N, PZ, SLES_LES_VREMAN>), gz, bz, 0, s, W, C, scale, coeff, part, tw, mb);
     } else {
       if (!skip_xinv) {
         ProfScope ps_(PK_XINV, s);
-        hipLaunchKernelGGL((k_xinv<T, N, 0, PS>), gs, bs, 0, s, uh, W, C, tw, xcd);
+        hipLaunchKernelGGL((k_xinv<T, N, 0, PS>), gs, bs, 0, s, uh, W, C, tw, xcd, mb);
       }
-      { ProfScope ps_(PK_YINV, s); hipLaunchKernelGGL((k_yinv<T, N, 0, PS>), gs, bs, 0, s, W, C, tw, xcd); }
+      { ProfScope ps_(PK_YINV, s); hipLaunchKernelGGL((k_yinv<T, N, 0, PS>), gs, bs, 0, s, W, C, tw, xcd, mb); }
       ProfScope ps_(PK_Z, s);
-      hipLaunchKernelGGL((k_zns<T, N, PZ>), dim3(ZB), dim3(ZLines<PZ>::THREADS), 0, s, W, C, scale, part, tw);
+      hipLaunchKernelGGL((k_zns<T, N, PZ>), gz, bz, 0, s, W, C, scale, part, tw, mb);
     }
     return check_launch("fused stage (inverse)") ? -1 : 0;
   }
@@ -1075,36 +1119,36 @@ struct Fused {
   // fuse_next: also run the next stage's x-inverse (RK4 stages 0-2)
   // fuse_phys (last RK4 stage): also run the physical-output pass's x-inverse
   static int finish(const sles_plan_s* p, int les, cpx<T>* uh, cpx<T>* uh0, cpx<T>* uh1, cpx<T>* aux, cpx<T>* W,
-                    const StepArgs& a, hipStream_t s, bool fuse_next = false, bool fuse_phys = false) {
+                    const StepArgs& a, hipStream_t s, bool fuse_next, bool fuse_phys, const Members& mb, int nm) {
     const long C = ncplx(p);
     const cpx<T>* tw = (const cpx<T>*)p->tw;
     const int xcd = p->tune_xcd;
     constexpr long NCOL = (long)N * (N / 2 + 1);  // columns of every strided pass
-    const dim3 gs((unsigned)((NCOL + SS::TK - 1) / SS::TK)), bs(SS::THREADS);
-    const dim3 gx((unsigned)((NCOL + SX::TK - 1) / SX::TK)), bx(SX::THREADS);
+    const dim3 gs((unsigned)((NCOL + SS::TK - 1) / SS::TK), (unsigned)nm), bs(SS::THREADS);
+    const dim3 gx((unsigned)((NCOL + SX::TK - 1) / SX::TK), (unsigned)nm), bx(SX::THREADS);
     if (les) {
-      { ProfScope ps_(PK_YFWD, s); hipLaunchKernelGGL((k_yfwd<T, N, 1, PS>), gs, bs, 0, s, W, C, tw, xcd); }
+      { ProfScope ps_(PK_YFWD, s); hipLaunchKernelGGL((k_yfwd<T, N, 1, PS>), gs, bs, 0, s, W, C, tw, xcd, mb); }
       ProfScope ps_(PK_XFWD, s);
       if constexpr (FUSE_X) {
         if (fuse_next) {
-          hipLaunchKernelGGL((k_xfwd<T, N, 1, PX, true>), gx, bx, 0, s, W, C, uh, uh0, uh1, aux, a, tw, xcd);
+          hipLaunchKernelGGL((k_xfwd<T, N, 1, PX, true>), gx, bx, 0, s, W, C, uh, uh0, uh1, aux, a, tw, xcd, mb);
           return check_launch("fused stage (forward)");
         }
         if (fuse_phys) {
-          hipLaunchKernelGGL((k_xfwd<T, N, 1, PX, true, 0>), gx, bx, 0, s, W, C, uh, uh0, uh1, aux, a, tw, xcd);
+          hipLaunchKernelGGL((k_xfwd<T, N, 1, PX, true, 0>), gx, bx, 0, s, W, C, uh, uh0, uh1, aux, a, tw, xcd, mb);
           return check_launch("fused stage (forward)");
         }
       }
-      hipLaunchKernelGGL((k_xfwd<T, N, 1, PX>), gx, bx, 0, s, W, C, uh, uh0, uh1, aux, a, tw, xcd);
+      hipLaunchKernelGGL((k_xfwd<T, N, 1, PX>), gx, bx, 0, s, W, C, uh, uh0, uh1, aux, a, tw, xcd, mb);
     } else {
-      { ProfScope ps_(PK_YFWD, s); hipLaunchKernelGGL((k_yfwd<T, N, 0, PS>), gs, bs, 0, s, W, C, tw, xcd); }
+      { ProfScope ps_(PK_YFWD, s); hipLaunchKernelGGL((k_yfwd<T, N, 0, PS>), gs, bs, 0, s, W, C, tw, xcd, mb); }
       ProfScope ps_(PK_XFWD, s);
       if constexpr (FUSE_X)
         if (fuse_next || fuse_phys) {  // the same output layout
-          hipLaunchKernelGGL((k_xfwd<T, N, 0, PX, true>), gx, bx, 0, s, W, C, uh, uh0, uh1, aux, a, tw, xcd);
+          hipLaunchKernelGGL((k_xfwd<T, N, 0, PX, true>), gx, bx, 0, s, W, C, uh, uh0, uh1, aux, a, tw, xcd, mb);
           return check_launch("fused stage (forward)");
         }
-      hipLaunchKernelGGL((k_xfwd<T, N, 0, PX>), gx, bx, 0, s, W, C, uh, uh0, uh1, aux, a, tw, xcd);
+      hipLaunchKernelGGL((k_xfwd<T, N, 0, PX>), gx, bx, 0, s, W, C, uh, uh0, uh1, aux, a, tw, xcd, mb);
     }
     return check_launch("fused stage (forward)");
   }
@@ -1118,8 +1162,8 @@ struct Fused {
     constexpr long NCOL = (long)N * (N / 2 + 1);  // columns of every strided pass
     const dim3 gs((unsigned)((NCOL + SS::TK - 1) / SS::TK)), bs(SS::THREADS);
     ProfScope ps_(PK_PHYS, s);
-    if (!skip_xinv) hipLaunchKernelGGL((k_xinv<T, N, 0, PS>), gs, bs, 0, s, uh, W, C, tw, xcd);
-    hipLaunchKernelGGL((k_yinv<T, N, 0, PS>), gs, bs, 0, s, W, C, tw, xcd);
+    if (!skip_xinv) hipLaunchKernelGGL((k_xinv<T, N, 0, PS>), gs, bs, 0, s, uh, W, C, tw, xcd, Members{});
+    hipLaunchKernelGGL((k_yinv<T, N, 0, PS>), gs, bs, 0, s, W, C, tw, xcd, Members{});
     if (check_launch("fused physical")) return -1;
     const double sc = 1.0 / ((double)N * N * N);
     if (u_out && z_c2r_rows(p, W, u_out, 3L * N * N, sc, s)) return -1;
@@ -1131,8 +1175,10 @@ struct Fused {
 template <typename T, int N, class PS, class PX, class PZ>
 int step_impl(const sles_plan_s* p, int scheme, const sles_step_params* prm, void* uh_, void* uh0_, void* uh1_,
               void* aux_, void* u_out, void* curl_out, double* scalars, void* work, hipStream_t s, void* stages,
-              const DriveArgs* drive) {
+              const DriveArgs* drive, const Batch* batch) {
   using F = Fused<T, N, PS, PX, PZ>;
+  const int nm = batch ? batch->count : 1;
+  const Members mb = batch ? batch->mb : Members{0, 0, 0, nullptr};
   cpx<T>* uh = (cpx<T>*)uh_;
   cpx<T>* uh0 = (cpx<T>*)uh0_;
   cpx<T>* uh1 = (cpx<T>*)uh1_;
@@ -1170,7 +1216,7 @@ int step_impl(const sles_plan_s* p, int scheme, const sles_step_params* prm, voi
     double* pp = (ddt && st == 0) ? part : nullptr;
     const bool fuse = F::FUSE_X && p->tune_fuse_x && scheme == SLES_SCHEME_RK4;
     const bool fused_in = fuse && st > 0;  // x-inverse ran in the previous x-forward
-    if (F::stage(p, les, coeff, uh, W, pp, s, fused_in)) return -1;
+    if (F::stage(p, les, coeff, uh, W, pp, s, fused_in, mb, nm)) return -1;
     if (pp) {
       // two levels: up to 64 chunks of the per-block maxima, then one block
       // (the partials area holds N^2 + 64 doubles; the z grid is at most N^2)
@@ -1178,16 +1224,18 @@ int step_impl(const sles_plan_s* p, int scheme, const sles_step_params* prm, voi
       constexpr int NB = F::ZB < 64 ? F::ZB : 64;
       constexpr int CHUNK = (F::ZB + NB - 1) / NB;
       double* part2 = part + F::ZB;
-      hipLaunchKernelGGL(k_nanmax_chunks, dim3(NB), dim3(256), 0, s, (const double*)part, F::ZB, CHUNK, part2);
+      hipLaunchKernelGGL(k_nanmax_chunks, dim3(NB, nm), dim3(256), 0, s, (const double*)part, F::ZB, CHUNK, part2,
+                         mb);
       if (drive)
-        hipLaunchKernelGGL(k_cfl_drive, dim3(1), dim3(64), 0, s, (const double*)part2, NB, prm->cfl, drive->cfl_first,
-                           prm->dx, prm->nu, drive->state);
+        hipLaunchKernelGGL(k_cfl_drive, dim3(1, nm), dim3(64), 0, s, (const double*)part2, NB, prm->cfl,
+                           drive->cfl_first, prm->dx, prm->nu, drive->state, mb);
       else
         hipLaunchKernelGGL(k_cfl, dim3(1), dim3(64), 0, s, (const double*)part2, NB, prm->cfl, prm->dx, prm->nu,
                            scalars);
       if (check_launch("cfl")) return -1;
     }
-    if (F::finish(p, les, uh, uh0, uh1, aux, W, a, s, fuse && st < 3, fuse && st == 3 && (u_out || curl_out)))
+    if (F::finish(p, les, uh, uh0, uh1, aux, W, a, s, fuse && st < 3, fuse && st == 3 && (u_out || curl_out), mb,
+                  nm))
       return -1;
   }
   // the last fused RK4 x-forward already ran the physical pass's x-inverse
@@ -1209,7 +1257,7 @@ struct HasP4 {
 template <typename T, int N>
 int step_t(const sles_plan_s* p, int scheme, const sles_step_params* prm, void* uh, void* uh0, void* uh1, void* aux,
            void* u_out, void* curl_out, double* scalars, void* work, hipStream_t s, void* stages,
-           const DriveArgs* drive) {
+           const DriveArgs* drive, const Batch* batch) {
   using D = LinePlan<N>;
   if constexpr (HasP4<N>::value) {
     using Q = LinePlan4<N>;
@@ -1217,7 +1265,7 @@ int step_t(const sles_plan_s* p, int scheme, const sles_step_params* prm, void* 
 #define SLES_SEL(K, A, B, Cz) \
   case K:                     \
     return step_impl<T, N, A, B, Cz>(p, scheme, prm, uh, uh0, uh1, aux, u_out, curl_out, scalars, work, s, stages, \
-                                     drive);
+                                     drive, batch);
     switch (sel) {
       SLES_SEL(0, D, D, D)
       SLES_SEL(1, Q, D, D)
@@ -1231,15 +1279,15 @@ int step_t(const sles_plan_s* p, int scheme, const sles_step_params* prm, void* 
 #undef SLES_SEL
   }
   return step_impl<T, N, D, D, D>(p, scheme, prm, uh, uh0, uh1, aux, u_out, curl_out, scalars, work, s, stages,
-                                  drive);
+                                  drive, batch);
 }
 
 // Per-precision instantiation (sles_rhs_f64.hip / sles_rhs_f32.hip).
 int step_f64(const sles_plan_s* p, int scheme, const sles_step_params* prm, void* uh, void* uh0, void* uh1,
              void* aux, void* u_out, void* curl_out, double* scalars, void* work, hipStream_t s, void* stages,
-             const DriveArgs* drive = nullptr);
+             const DriveArgs* drive = nullptr, const Batch* batch = nullptr);
 int step_f32(const sles_plan_s* p, int scheme, const sles_step_params* prm, void* uh, void* uh0, void* uh1,
              void* aux, void* u_out, void* curl_out, double* scalars, void* work, hipStream_t s, void* stages,
-             const DriveArgs* drive = nullptr);
+             const DriveArgs* drive = nullptr, const Batch* batch = nullptr);
 
 }  // namespace sles

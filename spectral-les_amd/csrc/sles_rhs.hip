@@ -131,6 +131,52 @@ int sles_step_drive(sles_plan p, int scheme, const sles_step_params* prm, double
   return 0;
 }
 
+// Members of a batch lie at fixed strides: state fields packed (3 C each),
+// workspaces rounded up to 256 bytes, driver states SLES_DRIVE_STATE_SIZE
+// doubles.
+static size_t batch_work_stride(sles_plan p, int les_model) {
+  return (sles_step_work_bytes(p, les_model) + 255) / 256 * 256;
+}
+
+size_t sles_step_batch_work_bytes(sles_plan p, int les_model, int members) {
+  if (!p || members < 1) return 0;
+  return batch_work_stride(p, les_model) * (size_t)members;
+}
+
+int sles_step_drive_batch(sles_plan p, int scheme, const sles_step_params* prm, int members, const double* sgs_coeff,
+                          double cfl_first, int nsteps, void* u_hat, void* u_hat0, void* u_hat1, void* aux,
+                          double* state, void* work, void* stream) {
+  if (!p || !prm || !u_hat || !state || !work) return fail("sles_step_drive_batch: null argument");
+  if (scheme != SLES_SCHEME_RK4 && scheme != SLES_SCHEME_RK3 && scheme != SLES_SCHEME_ETD2)
+    return fail("sles_step_drive_batch: scheme must be RK4, RK3 or ETD2");
+  if (prm->les_model < 0 || prm->les_model > 2) return fail("sles_step_drive_batch: les_model must be 0, 1 or 2");
+  if (members < 1 || members > SLES_BATCH_MAX_MEMBERS)
+    return fail("sles_step_drive_batch: members must be in [1, " + std::to_string(SLES_BATCH_MAX_MEMBERS) + "]");
+  if (prm->les_model && !sgs_coeff) return fail("sles_step_drive_batch: LES needs the per-member sgs_coeff");
+  if (scheme != SLES_SCHEME_ETD2 && !u_hat0) return fail("sles_step_drive_batch: RK schemes need u_hat0");
+  if (scheme == SLES_SCHEME_RK4 && !u_hat1) return fail("sles_step_drive_batch: RK4 needs u_hat1");
+  if (scheme == SLES_SCHEME_ETD2 && !aux) return fail("sles_step_drive_batch: ETD2 needs aux");
+  if (nsteps < 0) return fail("sles_step_drive_batch: nsteps must be >= 0");
+  if (!sles_supported_n(p->n)) return fail("sles_step_drive_batch: unsupported grid size");
+  const long esz = p->precision == SLES_FP64 ? 16 : 8;
+  Batch b{};
+  b.count = members;
+  b.mb.state = 3 * ncplx(p) * esz;
+  b.mb.work = (long)batch_work_stride(p, prm->les_model);
+  b.mb.drive = (long)(SLES_DRIVE_STATE_SIZE * sizeof(double));
+  b.mb.coeff = prm->les_model ? sgs_coeff : nullptr;
+  hipStream_t s = (hipStream_t)stream;
+  const DriveArgs drv{state, cfl_first};
+  for (int i = 0; i < nsteps; ++i) {
+    const int rc =
+        p->precision == SLES_FP64
+            ? step_f64(p, scheme, prm, u_hat, u_hat0, u_hat1, aux, nullptr, nullptr, nullptr, work, s, nullptr, &drv, &b)
+            : step_f32(p, scheme, prm, u_hat, u_hat0, u_hat1, aux, nullptr, nullptr, nullptr, work, s, nullptr, &drv, &b);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
 int sles_rk4_stage_states(sles_plan p, const sles_step_params* prm, void* u_hat, void* u_hat0, void* u_hat1,
                           void* stages, void* work, void* stream) {
   if (!p || !prm || !u_hat || !u_hat0 || !u_hat1 || !stages || !work)

@@ -68,6 +68,8 @@ _PROTOS = {
     "sles_profile_read": (_int, [_vp, _vp, _int]),
     "sles_step": (_int, [_vp, _int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "sles_step_drive": (_int, [_vp, _int, _vp, _dbl, _int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "sles_step_batch_work_bytes": (_size, [_vp, _int, _int]),
+    "sles_step_drive_batch": (_int, [_vp, _int, _vp, _int, _vp, _dbl, _int, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "sles_stream_probe": (_int, [_vp, _vp, _int, _vp, _int, _int, _vp]),
     "sles_step_noalloc": (_int, [_vp, _int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "sles_rk4_stage_states": (_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),

@@ -3,8 +3,9 @@
 precision_study  FP32 form vs the FP64 form on the same seeded field and the
                  same dt sequence: velocity rel-L2, max relative E(k) error,
                  steps/s of each form.
-ensemble         many members of different grid sizes advanced together, one
-                 captured HIP graph per ensemble step (see ensemble.py).
+ensemble         many members of different grid sizes advanced together: one
+                 launch per pass per grid size, replays of one captured HIP
+                 graph (see ensemble.py); --per-member for one call per member.
 ensemble-adjoint d(loss)/dCs of every member through the hand-written adjoint.
 differentiable   Vreman ETD2 gradients against finite differences.
 fft              generic 3D transforms (batch of 3 fields): ms and GB/s per call.
@@ -277,6 +278,27 @@ def adjoint_cost(N=256, scheme="RK4", les="Smagorinsky", precision="fp32", steps
     return res
 
 
+def ensemble_rate(groups=((64, 8), (96, 8), (128, 8)), steps=200, steps_per_graph=10, batched=True,
+                  precision="fp32", seed=0):
+    """Aggregate grid points x steps / s of an ensemble (Smagorinsky RK4,
+    Cs ~ U[0.10, 0.20], own seeds, CFL dt on the device) advanced by replays
+    of one captured graph of steps_per_graph ensemble steps; batched: one
+    launch per pass per grid size, else one sles_step_drive call per member.
+    One untimed replay first."""
+    from .ensemble import S_STATUS, S_STEPS, Ensemble
+
+    ens = Ensemble(groups, precision=precision, seed=seed, batched=batched)
+    ens.capture(steps_per_graph=steps_per_graph)
+    ens.graph.replay()  # untimed: first replay
+    torch.cuda.synchronize()
+    rate = ens.run(steps)
+    st = ens.states()
+    ok = bool((st[:, S_STATUS] == 0).all()) and bool((st[:, S_STEPS] == steps + steps_per_graph).all())
+    return {"groups": [list(g) for g in groups], "members": [[d.N, d.Cs] for d in ens.members], "steps": steps,
+            "steps_per_graph": steps_per_graph, "batched": batched, "precision": precision,
+            "grid_points_x_steps_per_s": rate, "all_members_stepped": ok, "dt_last": ens.dt_values()}
+
+
 def ensemble_adjoint(groups=((64, 8), (96, 8), (128, 8)), steps=200, checkpoint_every=10, nu=1e-4, seed=0,
                      cs_range=(0.10, 0.20), fd_check=True):
     """d(loss)/dCs for every member of the ensemble after `steps` RK4 steps
@@ -447,6 +469,7 @@ def main(argv=None):
     r = sub.add_parser("ensemble")
     r.add_argument("--steps", type=int, default=200)
     r.add_argument("--steps-per-graph", type=int, default=10)
+    r.add_argument("--per-member", action="store_true", help="one sles_step_drive call per member")
     r.add_argument("--out", default=None)
     pt = sub.add_parser("precision-tgv")
     pt.add_argument("--n", type=int, default=64)
@@ -497,16 +520,9 @@ def main(argv=None):
                 json.dump(res, f, indent=1)
         return
     if a.cmd == "ensemble":
-        from .ensemble import Ensemble
-
-        ens = Ensemble(((64, 8), (96, 8), (128, 8)), precision="fp32", seed=0)
-        ens.capture(steps_per_graph=a.steps_per_graph)
-        rate = ens.run(a.steps)
-        res = {"members": [[d.N, d.Cs] for d in ens.members], "steps": a.steps,
-               "steps_per_graph": a.steps_per_graph, "grid_points_x_steps_per_s": rate,
-               "dt_last": ens.dt_values()}
-        print(json.dumps({"grid_points_x_steps_per_s": rate, "steps": a.steps,
-                          "steps_per_graph": a.steps_per_graph}), flush=True)
+        res = ensemble_rate(steps=a.steps, steps_per_graph=a.steps_per_graph, batched=not a.per_member)
+        print(json.dumps({k: res[k] for k in ("grid_points_x_steps_per_s", "steps", "steps_per_graph", "batched")}),
+              flush=True)
         if a.out:
             with open(a.out, "w") as f:
                 json.dump(res, f, indent=1)

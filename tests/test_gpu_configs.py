@@ -176,18 +176,20 @@ def _oracle_member(d, U_hat0):
 
 
 def test_ensemble_members_match_oracle():
-    """One member each of 64^3, 96^3, 128^3 (FP32 form, own seed and Cs): the
-    first step with the host CFL dt (0.3), the second eager and the third as a
-    replay of the captured graph, both with the device CFL dt (0.5); each
+    """One member each of 64^3, 96^3, 128^3 (FP32 form, own seed and Cs),
+    advanced by three replays of the captured graph of batched launches: the
+    first step at the CFL dt 0.3, the next ones at 0.5, all on the device; each
     member against the FP64 oracle run with its own CFL dt."""
-    from sles.ensemble import Ensemble
+    from sles.ensemble import S_DT_APPLY, S_STEPS, Ensemble
 
     ens = Ensemble(((64, 1), (96, 1), (128, 1)), precision="fp32", seed=7)
     init = [d.U_hat.to(torch.complex128).cpu().numpy() for d in ens.members]
-    ens.capture(steps_per_graph=1)  # steps 1 (host dt) and 2 (eager, device dt)
-    ens.graph.replay()  # step 3
+    ens.capture(steps_per_graph=1)
+    for _ in range(3):
+        ens.graph.replay()
     torch.cuda.synchronize()
-    for d, U0 in zip(ens.members, init):
+    st = ens.states()
+    for i, (d, U0) in enumerate(zip(ens.members, init)):
         o = _oracle_member(d, U0)
         o.U_hat1[:] = o.U_hat
         for k in range(3):
@@ -197,7 +199,8 @@ def test_ensemble_members_match_oracle():
         print(f"[ensemble] member {d.N}^3 Cs={d.Cs:.4f}: FP32 vs FP64 oracle after 3 steps rel-L2 {r:.3e}",
               flush=True)
         assert r < 1e-5
-        assert math.isclose(float(d._scalars[1]), o.dt, rel_tol=1e-5)  # the device dt of step 3
+        assert st[i, S_STEPS] == 3
+        assert math.isclose(float(st[i, S_DT_APPLY]), o.dt, rel_tol=1e-5)  # the device dt of step 3
 
 
 def test_ensemble_member_adjoint_dcs_vs_fd():
@@ -234,6 +237,8 @@ def test_full_ensemble_graph_replays():
     energy (nu > 0, eddy viscosity >= 0, no forcing)."""
     from sles.ensemble import Ensemble
 
+    from sles.ensemble import S_DT_APPLY, S_STATUS, S_STEPS
+
     ens = Ensemble(((64, 8), (96, 8), (128, 8)), precision="fp32", seed=0)
     assert len(ens.members) == 24 and len({d.Cs for d in ens.members}) == 24
     ens.capture(steps_per_graph=1)
@@ -242,9 +247,11 @@ def test_full_ensemble_graph_replays():
     for _ in range(3):
         ens.graph.replay()
     torch.cuda.synchronize()
-    for d, e0 in zip(ens.members, before):
+    st = ens.states()
+    for i, (d, e0) in enumerate(zip(ens.members, before)):
         assert bool(torch.isfinite(torch.view_as_real(d.U_hat)).all()), d.N
         div, e = _div_energy(d.U_hat, d.N)
         assert div < 1e-5, (d.N, div)
         assert e < e0, (d.N, e, e0)
-        assert float(d._scalars[1]) >= 1e-6  # the device dt guard did not skip a step
+        # every member took its 3 steps (the device dt guard skipped none)
+        assert st[i, S_STEPS] == 3 and st[i, S_STATUS] == 0 and st[i, S_DT_APPLY] >= 1e-6
