@@ -175,9 +175,27 @@ int sles_max(sles_plan plan, void* const f[], int nfields, double* out, void* wo
 /* Binned spectrum of calculate_Ek (src/SM_solver.py:154-192): for bins
  * b = 0..nbins-1 with K^2 in [(5b)^2, (5b+5)^2):
  *   ek_sum[b] = sum Ek, ksum[b] = sum sqrt(K^2), count[b] = number of modes,
- * Ek = sum_i |u_hat_i|^2, doubled for kz >= 1.  u_hat = 3 contiguous fields. */
+ * Ek = sum_i |u_hat_i|^2, doubled for kz >= 1.  u_hat = 3 contiguous fields.
+ * Deterministic: the same input gives the same bits on every call (fixed
+ * summation order, no atomics).  nbins <= 640 (every mode of N = 512 falls in
+ * 91 bins).  `work` holds sles_spectrum_work_bytes(plan, nbins) bytes. */
+size_t sles_spectrum_work_bytes(sles_plan plan, int nbins);
 int sles_spectrum(sles_plan plan, const void* u_hat, int nbins, double* ek_sum, double* ksum,
-                  double* count, void* stream);
+                  double* count, void* work, void* stream);
+
+/* Spectrum over unit shells, the E(k) of a spectral loss:
+ * e[k-1] = sum over modes with |K| in [k - 1/2, k + 1/2) of
+ * (1/2) w sum_i |u_hat_i|^2 / N^6, k = 1..kmax, w = 1 on the kz = 0 and
+ * kz = N/2 planes and 2 elsewhere, kmax <= 640 (N = 512 has 443 shells).
+ * Deterministic as sles_spectrum; `work` as for sles_spectrum with
+ * nbins = kmax.  sles_shell_spectrum_vjp writes the
+ * gradient of sum_k grad_e[k-1] e[k-1] with respect to u_hat (as
+ * d/dRe + i d/dIm, 3 fields like u_hat): grad_e[k-1] w u_hat / N^6 in shells
+ * 1..kmax, 0 elsewhere.  No counterpart in the reference (the loss of
+ * differentiable runs). */
+int sles_shell_spectrum(sles_plan plan, const void* u_hat, int kmax, double* e, void* work, void* stream);
+int sles_shell_spectrum_vjp(sles_plan plan, const void* u_hat, int kmax, const double* grad_e, void* grad_u,
+                            void* stream);
 
 /* ---- fused time step (src/RK4_method.py:20-71, src/RK3_method.py:20-41,
  * src/ETD2_method.py:7-37 with src/smagorinsky.py / src/vreman.py) ----

@@ -7,6 +7,7 @@
 // fused time-step path (sles_rhs.hip) does not use them.
 #include <hip/hip_runtime.h>
 
+#include <climits>
 #include <cmath>
 #include <string>
 
@@ -311,45 +312,170 @@ __global__ void __launch_bounds__(kThreads) k_reduce_final(const double* partial
   if (threadIdx.x == 0) out[0] = acc;
 }
 
-constexpr int kMaxBins = 256;
+constexpr int kMaxBins = 640;  // LDS of the binning kernel: 4 waves x 3 x nbins doubles (60 KiB at most)
 
-__global__ void k_zero(double* a, int n) {
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) a[i] = 0.0;
+// ---------------------------------------------------------------------------
+// Binned spectra, deterministic and without atomics.  Modes are visited row
+// by row ((kx, ky) rows of N/2+1 kz values); block b owns the rows
+// [b rpb, (b+1) rpb) and its wave w every fourth of them, 64 consecutive kz
+// per step.  Along a row K^2 grows with kz, so a step's bins are
+// non-decreasing across the wave's lanes: a segmented inclusive scan keyed by
+// the bin (shuffles) leaves each bin's sum in the last lane of its run, which
+// adds it to the wave's own LDS bins.  The block then sums its four waves in
+// order into its partial row, and a second kernel sums the partial rows in a
+// fixed order: the result does not depend on scheduling.
+//   RULE_EK    calculate_Ek (src/SM_solver.py:154-192): bin = isqrt(K^2) / 5,
+//              value sum_i |u_i|^2 (numpy.absolute), doubled for kz >= 1
+//   RULE_SHELL unit shells |K| in [k - 1/2, k + 1/2), k = 1..nbins (bin k-1),
+//              value (1/2) w sum_i |u_i|^2 / N^6, w = 1 on the kz = 0 and
+//              kz = N/2 planes and 2 elsewhere (the rfft doubling)
+// Partials: [block][3][nbins] doubles (value, sum of |K|, mode count).
+// ---------------------------------------------------------------------------
+enum { RULE_EK = 0, RULE_SHELL = 1 };
+constexpr int kBinWaves = kThreads / 64;
+constexpr int kBinMaxBlocks = 1024;
+
+// rows per block of the binned spectra: at least 4 (one per wave), at most
+// kBinMaxBlocks blocks
+static inline long bin_rows_per_block(int n) {
+  const long rows = (long)n * n;
+  long r = (rows + kBinMaxBlocks - 1) / kBinMaxBlocks;
+  return r < kBinWaves ? kBinWaves : r;
+}
+static inline long bin_blocks(int n) {
+  const long rows = (long)n * n, r = bin_rows_per_block(n);
+  return (rows + r - 1) / r;
 }
 
-template <typename T>
-__global__ void __launch_bounds__(kThreads) k_spectrum(const cpx<T>* uh, long nc, int n, int nh, int nbins,
-                                                       double* ek, double* ks, double* cnt) {
-  __shared__ double s_ek[kMaxBins], s_k[kMaxBins], s_c[kMaxBins];
-  for (int i = threadIdx.x; i < nbins; i += blockDim.x) s_ek[i] = s_k[i] = s_c[i] = 0.0;
-  block_sync();
-  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < nc; e += (long)gridDim.x * blockDim.x) {
-    const KIndex k = kindex(e, n, nh);
-    const long k2 = (long)k.kx * k.kx + (long)k.ky * k.ky + (long)k.kz * k.kz;
+// bin of a mode, or -1 (below the first bin) / INT_MAX (beyond the last):
+// both keep the bins of a row non-decreasing in kz
+template <int RULE>
+__device__ __forceinline__ int mode_bin(long k2, int nbins) {
+  if (RULE == RULE_EK) {
     long r = (long)sqrt((double)k2);
     while (r * r > k2) --r;
     while ((r + 1) * (r + 1) <= k2) ++r;
-    const int bin = (int)(r / 5);
-    if (bin >= nbins) continue;
-    double v = 0.0;
+    const long b = r / 5;
+    return b < nbins ? (int)b : INT_MAX;
+  } else {
+    const long sh = (long)floor(sqrt((double)k2) + 0.5);
+    if (sh < 1) return -1;
+    return sh <= nbins ? (int)(sh - 1) : INT_MAX;
+  }
+}
+
+template <typename T, int RULE>
+__global__ void __launch_bounds__(kThreads) k_bins(const cpx<T>* __restrict__ uh, long nc, int n, int nh, int nbins,
+                                                   long rpb, double* __restrict__ part) {
+  extern __shared__ double s[];  // [wave][3][nbins]
+  for (int i = threadIdx.x; i < kBinWaves * 3 * nbins; i += kThreads) s[i] = 0.0;
+  block_sync();
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const long rows = (long)n * n;
+  const long r0 = (long)blockIdx.x * rpb, r1 = r0 + rpb < rows ? r0 + rpb : rows;
+  const double c0 = 0.5 / ((double)n * n * n * (double)n * n * n);
+  for (long row = r0 + w; row < r1; row += kBinWaves) {
+    const int kx = kfreq((int)(row / n), n), ky = kfreq((int)(row % n), n);
+    const long q = (long)kx * kx + (long)ky * ky;
+    for (int z0 = 0; z0 < nh; z0 += 64) {
+      const int kz = z0 + lane;
+      const long k2 = q + (long)kz * kz;
+      int bin = kz < nh ? mode_bin<RULE>(k2, nbins) : INT_MAX;
+      // the rest of the row lies beyond the last bin (bins grow with kz)
+      if (__ballot(bin != INT_MAX) == 0) break;
+      double v = 0.0, kk = 0.0, c = 0.0;
+      if (bin >= 0 && bin < nbins) {
+        const long e = row * nh + kz;
+        if (RULE == RULE_EK) {
+#pragma unroll
+          for (int i = 0; i < 3; ++i) {
+            const cpx<T> u = uh[i * nc + e];
+            const double m = hypot(fabs((double)u.x), fabs((double)u.y));  // numpy.absolute
+            v += m * m;
+          }
+          if (kz >= 1) v *= 2.0;
+        } else {
+          double sq[3];
+#pragma unroll
+          for (int i = 0; i < 3; ++i) {
+            const cpx<T> u = uh[i * nc + e];
+            sq[i] = (double)u.x * (double)u.x + (double)u.y * (double)u.y;
+          }
+          const double wgt = (kz == 0 || 2 * kz == n) ? 1.0 : 2.0;
+          v = (c0 * wgt) * ((sq[0] + sq[1]) + sq[2]);
+        }
+        kk = sqrt((double)k2);
+        c = 1.0;
+      }
+      // segmented inclusive scan over the run of equal bins ending at each lane
+#pragma unroll
+      for (int d = 1; d < 64; d <<= 1) {
+        const double v2 = __shfl_up(v, d, 64), kk2 = __shfl_up(kk, d, 64), c2 = __shfl_up(c, d, 64);
+        const int b2 = __shfl_up(bin, d, 64);
+        if (lane >= d && b2 == bin) {
+          v += v2;
+          kk += kk2;
+          c += c2;
+        }
+      }
+      const int bn = __shfl_down(bin, 1, 64);
+      if ((lane == 63 || bn != bin) && bin >= 0 && bin < nbins) {
+        double* sw = s + (long)w * 3 * nbins;
+        sw[bin] += v;
+        sw[nbins + bin] += kk;
+        sw[2 * nbins + bin] += c;
+      }
+    }
+  }
+  block_sync();
+  for (int i = threadIdx.x; i < 3 * nbins; i += kThreads) {
+    const int qq = i / nbins, b = i % nbins;
+    double acc = s[qq * nbins + b];
+#pragma unroll
+    for (int ww = 1; ww < kBinWaves; ++ww) acc += s[((long)ww * 3 + qq) * nbins + b];
+    part[((long)blockIdx.x * 3 + qq) * nbins + b] = acc;
+  }
+}
+
+// out_q[b] = sum over the partial rows (block q*nbins + b of the grid): each
+// thread a fixed stride of rows, then a fixed shuffle tree and the waves in
+// order
+__global__ void __launch_bounds__(kThreads) k_bins_final(const double* __restrict__ part, int nblocks, int nbins,
+                                                         double* o0, double* o1, double* o2) {
+  const int qq = blockIdx.x / nbins, b = blockIdx.x % nbins;
+  double* out = qq == 0 ? o0 : qq == 1 ? o1 : o2;
+  if (!out) return;
+  double acc = 0.0;
+  for (int r = threadIdx.x; r < nblocks; r += kThreads) acc += part[((long)r * 3 + qq) * nbins + b];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+  __shared__ double sw[kBinWaves];
+  if ((threadIdx.x & 63) == 0) sw[threadIdx.x >> 6] = acc;
+  block_sync();
+  if (threadIdx.x == 0) {
+    double t = sw[0];
+    for (int ww = 1; ww < kBinWaves; ++ww) t += sw[ww];
+    out[b] = t;
+  }
+}
+
+// Vector-Jacobian product of the unit-shell spectrum: g_u = gE[k-1] w u / N^6
+// for modes in shells 1..nbins (the gradient of (1/2) w |u|^2 / N^6 with
+// respect to (Re u, Im u), as Re + i Im), 0 elsewhere.
+template <typename T>
+__global__ void k_shell_vjp(const cpx<T>* __restrict__ uh, const double* __restrict__ gE, long nc, int n, int nh,
+                            int nbins, cpx<T>* __restrict__ gu) {
+  const double c1 = 1.0 / ((double)n * n * n * (double)n * n * n);
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < nc; e += (long)gridDim.x * blockDim.x) {
+    const KIndex k = kindex(e, n, nh);
+    const long k2 = (long)k.kx * k.kx + (long)k.ky * k.ky + (long)k.kz * k.kz;
+    const int bin = mode_bin<RULE_SHELL>(k2, nbins);
+    double g = 0.0;
+    if (bin >= 0 && bin < nbins) g = gE[bin] * ((k.kz == 0 || 2 * k.kz == n) ? 1.0 : 2.0) * c1;
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
       const cpx<T> u = uh[i * nc + e];
-      const double ax = fabs((double)u.x), ay = fabs((double)u.y);
-      const double m = hypot(ax, ay);  // numpy.absolute
-      v += m * m;
-    }
-    if (k.kz >= 1) v *= 2.0;
-    atomicAdd(&s_ek[bin], v);
-    atomicAdd(&s_k[bin], sqrt((double)k2));
-    atomicAdd(&s_c[bin], 1.0);
-  }
-  block_sync();
-  for (int i = threadIdx.x; i < nbins; i += blockDim.x) {
-    if (s_c[i] != 0.0) {
-      atomicAdd(&ek[i], s_ek[i]);
-      atomicAdd(&ks[i], s_k[i]);
-      atomicAdd(&cnt[i], s_c[i]);
+      gu[i * nc + e] = cpx<T>{(T)(g * (double)u.x), (T)(g * (double)u.y)};
     }
   }
 }
@@ -381,6 +507,20 @@ using namespace sles;
       __VA_ARGS__;                            \
     }                                         \
   } while (0)
+
+template <int RULE>
+static int binned(sles_plan p, const void* uh, int nbins, double* o0, double* o1, double* o2, void* work,
+                  hipStream_t s, const char* name) {
+  const long nc = ncplx(p);
+  const long rpb = bin_rows_per_block(p->n);
+  const int nb = (int)bin_blocks(p->n);
+  double* part = (double*)work;
+  const size_t lds = sizeof(double) * kBinWaves * 3 * (size_t)nbins;
+  SLES_DISPATCH(p, hipLaunchKernelGGL((k_bins<T, RULE>), dim3(nb), dim3(kThreads), lds, s, (const cpx<T>*)uh, nc,
+                                      p->n, p->nh, nbins, rpb, part));
+  hipLaunchKernelGGL(k_bins_final, dim3(3 * nbins), dim3(kThreads), 0, s, (const double*)part, nb, nbins, o0, o1, o2);
+  return check_launch(name);
+}
 
 extern "C" {
 
@@ -529,17 +669,32 @@ int sles_max(sles_plan p, void* const f[], int nf, double* out, void* work, void
   return reduce_common(p, 1, f, nullptr, nf, out, work, stream, "sles_max");
 }
 
-int sles_spectrum(sles_plan p, const void* uh, int nbins, double* ek, double* ks, double* cnt, void* stream) {
-  if (!p || !uh || !ek || !ks || !cnt) return fail("sles_spectrum: null argument");
-  if (nbins < 1 || nbins > kMaxBins) return fail("sles_spectrum: nbins must be in [1, 256]");
-  hipStream_t s = (hipStream_t)stream;
-  hipLaunchKernelGGL(k_zero, dim3(1), dim3(kThreads), 0, s, ek, nbins);
-  hipLaunchKernelGGL(k_zero, dim3(1), dim3(kThreads), 0, s, ks, nbins);
-  hipLaunchKernelGGL(k_zero, dim3(1), dim3(kThreads), 0, s, cnt, nbins);
+size_t sles_spectrum_work_bytes(sles_plan p, int nbins) {
+  if (!p || nbins < 1) return 0;
+  return sizeof(double) * 3 * (size_t)nbins * (size_t)bin_blocks(p->n);
+}
+
+int sles_spectrum(sles_plan p, const void* uh, int nbins, double* ek, double* ks, double* cnt, void* work,
+                  void* stream) {
+  if (!p || !uh || !ek || !ks || !cnt || !work) return fail("sles_spectrum: null argument");
+  if (nbins < 1 || nbins > kMaxBins) return fail("sles_spectrum: nbins must be in [1, 640]");
+  return binned<RULE_EK>(p, uh, nbins, ek, ks, cnt, work, (hipStream_t)stream, "spectrum");
+}
+
+int sles_shell_spectrum(sles_plan p, const void* uh, int kmax, double* e, void* work, void* stream) {
+  if (!p || !uh || !e || !work) return fail("sles_shell_spectrum: null argument");
+  if (kmax < 1 || kmax > kMaxBins) return fail("sles_shell_spectrum: kmax must be in [1, 640]");
+  return binned<RULE_SHELL>(p, uh, kmax, e, nullptr, nullptr, work, (hipStream_t)stream, "shell_spectrum");
+}
+
+int sles_shell_spectrum_vjp(sles_plan p, const void* uh, int kmax, const double* grad_e, void* grad_u,
+                            void* stream) {
+  if (!p || !uh || !grad_e || !grad_u) return fail("sles_shell_spectrum_vjp: null argument");
+  if (kmax < 1 || kmax > kMaxBins) return fail("sles_shell_spectrum_vjp: kmax must be in [1, 640]");
   const long nc = ncplx(p);
-  SLES_DISPATCH(p, hipLaunchKernelGGL(k_spectrum<T>, dim3(kRedBlocks), dim3(kThreads), 0, s, (const cpx<T>*)uh, nc,
-                                      p->n, p->nh, nbins, ek, ks, cnt));
-  return check_launch("spectrum");
+  SLES_DISPATCH(p, hipLaunchKernelGGL(k_shell_vjp<T>, dim3(grid_for(nc)), dim3(kThreads), 0, (hipStream_t)stream,
+                                      (const cpx<T>*)uh, grad_e, nc, p->n, p->nh, kmax, (cpx<T>*)grad_u));
+  return check_launch("shell_spectrum_vjp");
 }
 
 }  // extern "C"

@@ -92,6 +92,7 @@ class Ek_plot_variables:
         self.temp_k = np.zeros(nb, dtype=float)
         self.temp_Ek = np.zeros(nb, dtype=float)
         self._dev = torch.zeros(3, nb, dtype=torch.float64, device=domain.device)
+        self._work = None  # binning partials, allocated by the first calculate_Ek
         if path is not None and os.path.exists(path):
             os.remove(path)
 
@@ -105,8 +106,11 @@ def calculate_Ek(domain, E):
     uh = domain.U_hat
     if not uh.is_contiguous():
         uh = uh.contiguous()
+    if E._work is None:
+        E._work = torch.empty(int(_lib.lib.sles_spectrum_work_bytes(domain.plan.handle, nb)), dtype=torch.uint8,
+                              device=domain.device)
     _lib.call("sles_spectrum", domain.plan.handle, uh.data_ptr(), nb, dev[0].data_ptr(), dev[1].data_ptr(),
-              dev[2].data_ptr(), domain.stream)
+              dev[2].data_ptr(), E._work.data_ptr(), domain.stream)
     ek, ks, cnt = dev.cpu().numpy()
     E.temp_k[:] = 0
     E.temp_Ek[:] = 0

@@ -53,7 +53,10 @@ _PROTOS = {
     "sles_sum_squares": (_int, [_vp, _vp, _int, _vp, _vp, _vp]),
     "sles_dot": (_int, [_vp, _vp, _vp, _int, _vp, _vp, _vp]),
     "sles_max": (_int, [_vp, _vp, _int, _vp, _vp, _vp]),
-    "sles_spectrum": (_int, [_vp, _vp, _int, _vp, _vp, _vp, _vp]),
+    "sles_spectrum_work_bytes": (_size, [_vp, _int]),
+    "sles_spectrum": (_int, [_vp, _vp, _int, _vp, _vp, _vp, _vp, _vp]),
+    "sles_shell_spectrum": (_int, [_vp, _vp, _int, _vp, _vp, _vp]),
+    "sles_shell_spectrum_vjp": (_int, [_vp, _vp, _int, _vp, _vp, _vp]),
     "sles_scale": (_int, [_vp, _vp, _int, _dbl, _vp]),
     "sles_step_work_bytes": (_size, [_vp, _int]),
     "sles_adjoint_work_bytes": (_size, [_vp, _int]),

@@ -284,40 +284,61 @@ class _Advance(torch.autograd.Function):
         return lam.to(ctx.in_dtype), gCs, None
 
 
-_SHELLS = {}
+_SPEC = {}
 
 
-def _shell_layout(N, kmax, device):
-    """Modes of the (N, N, N/2+1) half spectrum with unit-shell index
-    floor(|k| + 1/2) in 1..kmax, sorted by shell: (flat indices, rfft weights
-    w = 2 except on the kz = 0, N/2 planes, modes per shell).  Cached per
-    (N, kmax, device)."""
-    key = (N, kmax, str(device))
-    if key not in _SHELLS:
-        k = torch.fft.fftfreq(N, 1.0 / N, dtype=torch.float64, device=device)
-        kz = torch.arange(N // 2 + 1, dtype=torch.float64, device=device)
-        kmag = torch.sqrt(k[:, None, None] ** 2 + k[None, :, None] ** 2 + kz[None, None, :] ** 2)
-        shell = torch.floor(kmag + 0.5).to(torch.int64).flatten()
-        sel = ((shell >= 1) & (shell <= kmax)).nonzero().flatten()
-        order = torch.argsort(shell[sel], stable=True)
-        idx = sel[order]
-        w = torch.full((N, N, N // 2 + 1), 2.0, dtype=torch.float64, device=device)
-        w[..., 0] = 1.0
-        w[..., -1] = 1.0
-        counts = torch.bincount(shell[idx] - 1, minlength=kmax)
-        _SHELLS[key] = (idx, w.flatten()[idx].contiguous(), counts)
-    return _SHELLS[key]
+def _spectrum_plan(N, dtype, device, kmax):
+    """(plan, workspace) of the shell spectrum for (N, precision, device),
+    cached; the workspace grows to the largest kmax asked for."""
+    key = (N, dtype, str(device))
+    plan, work = _SPEC.get(key, (None, None))
+    if plan is None:
+        prec = _lib.SLES_FP64 if dtype == torch.complex128 else _lib.SLES_FP32
+        plan = _lib.Plan(N, prec, device)
+    nbytes = int(_lib.lib.sles_spectrum_work_bytes(plan.handle, kmax))
+    if work is None or work.numel() < nbytes:
+        work = torch.empty(nbytes, dtype=torch.uint8, device=device)
+    _SPEC[key] = (plan, work)
+    return plan, work
+
+
+class _ShellSpectrum(torch.autograd.Function):
+    """E(k) over unit shells (sles_shell_spectrum) with its vector-Jacobian
+    product (sles_shell_spectrum_vjp)."""
+
+    @staticmethod
+    def forward(ctx, U_hat, N, kmax):
+        U = U_hat.detach().contiguous()
+        plan, work = _spectrum_plan(N, U.dtype, U.device, kmax)
+        E = torch.empty(kmax, dtype=torch.float64, device=U.device)
+        _lib.call("sles_shell_spectrum", plan.handle, U.data_ptr(), kmax, E.data_ptr(), work.data_ptr(),
+                  _lib.stream_handle(U.device))
+        ctx.save_for_backward(U)
+        ctx.N, ctx.kmax = N, kmax
+        return E
+
+    @staticmethod
+    def backward(ctx, gE):
+        (U,) = ctx.saved_tensors
+        plan, _ = _spectrum_plan(ctx.N, U.dtype, U.device, ctx.kmax)
+        g = gE.detach().to(torch.float64).contiguous()
+        gU = torch.empty_like(U)
+        _lib.call("sles_shell_spectrum_vjp", plan.handle, U.data_ptr(), ctx.kmax, g.data_ptr(), gU.data_ptr(),
+                  _lib.stream_handle(U.device))
+        return gU, None, None
 
 
 def shell_spectrum(U_hat, N, kmax=40):
     """E(k) over unit shells |k| in [k - 1/2, k + 1/2), k = 1..kmax:
-    sum (1/2) w |U_hat|^2 / N^6 with the rfft doubling w.  Differentiable torch
-    ops: a gather of the modes sorted by shell and a segmented sum per shell
-    (no atomics), FP64."""
-    idx, w, counts = _shell_layout(N, kmax, U_hat.device)
-    sq = (U_hat.real.double() ** 2 + U_hat.imag.double() ** 2).sum(0).flatten()
-    e = (0.5 / float(N) ** 6) * w * sq[idx]
-    return torch.segment_reduce(e, "sum", lengths=counts)
+    sum (1/2) w |U_hat|^2 / N^6 with the rfft doubling w (1 on the kz = 0 and
+    N/2 planes, 2 elsewhere), FP64, differentiable.  One deterministic
+    binning kernel (segmented wavefront sums, no atomics) and its hand-written
+    vector-Jacobian product; U_hat (3, N, N, N/2+1) complex on the GPU."""
+    if not U_hat.is_cuda or U_hat.dtype not in (torch.complex64, torch.complex128):
+        raise ValueError("shell_spectrum: expected a complex64/complex128 CUDA tensor (3, N, N, N/2+1)")
+    if tuple(U_hat.shape) != (3, N, N, N // 2 + 1):
+        raise ValueError(f"shell_spectrum: expected shape (3, {N}, {N}, {N // 2 + 1}), got {tuple(U_hat.shape)}")
+    return _ShellSpectrum.apply(U_hat, int(N), int(kmax))
 
 
 def spectrum_loss(U_hat, target_E, N, kmax=40):
