@@ -45,8 +45,8 @@ int sles_plan_info(sles_plan plan, int* n, int* nh, int* precision);
  * sles/autotune.py, which persists its choices in a JSON table) and of the
  * generic transforms.  Defaults: 0 for the three plan choices, 1 for
  * SLES_TUNE_XCD and SLES_TUNE_FUSE_X; SLES_TUNE_FFT_WIDE 3 at N = 512,
- * 1 (FP64) / 2 (FP32) at N = 256, 0 otherwise.  No counterpart in the
- * reference.
+ * 1 (FP64) / 2 (FP32) at N = 256, 0 otherwise; SLES_TUNE_PAD 17 for
+ * N >= 128, 0 otherwise.  No counterpart in the reference.
  *   SLES_TUNE_STRIDED_RADIX4  x/y inverse + y forward passes use the radix-4
  *                             line plan (4 points per thread) instead of radix-8
  *   SLES_TUNE_XFWD_RADIX4     same for the x-forward + stage-update pass
@@ -62,6 +62,13 @@ int sles_plan_info(sles_plan plan, int* n, int* nh, int* precision);
  *                             twice the columns (256- or 512-byte row runs, LDS tile
  *                             up to 128 KiB); N = 256 and 512 only (ignored
  *                             otherwise); values 0..3
+ *   SLES_TUNE_PAD             fused step: the workspace's kx planes (and the
+ *                             copies of the RK state that sles_step_drive /
+ *                             sles_step_drive_batch step on) start every
+ *                             N (N/2+1) values + this many 128-byte lines
+ *                             (0..SLES_TUNE_PAD_MAX); 0 steps the caller's
+ *                             arrays in place.  sles_step_work_bytes depends
+ *                             on it: query it after setting this key
  * The "radix-4" alternative plans: 4 points per thread (radix 4) for
  * N = 32, 64, 128, 256, 512; 6 points per thread (radix 3, then 2) for
  * N = 12, 24, 48, 96, 192, 384; ignored for N = 8, 16. */
@@ -71,8 +78,10 @@ enum {
   SLES_TUNE_Z_RADIX4 = 2,
   SLES_TUNE_XCD = 3,
   SLES_TUNE_FFT_WIDE = 4,
-  SLES_TUNE_FUSE_X = 5
+  SLES_TUNE_FUSE_X = 5,
+  SLES_TUNE_PAD = 6
 };
+#define SLES_TUNE_PAD_MAX 64
 int sles_plan_set_tuning(sles_plan plan, int key, int value);
 int sles_plan_get_tuning(sles_plan plan, int key, int* value);
 
@@ -269,7 +278,11 @@ size_t sles_step_work_bytes(sles_plan plan, int les_model);
  * state the reference leaves); they are written by the call's last step only,
  * since no step reads them (each recomputes its physical fields from U_hat)
  * and nothing else runs between the steps of a call.  `work` as for
- * sles_step.  No counterpart in the
+ * sles_step; with SLES_TUNE_PAD > 0 the call copies U_hat (and ETD2's aux)
+ * into the workspace's padded layout, steps that copy and writes U_hat,
+ * U_hat0, U_hat1 and aux back on return (U_hat0, U_hat1 and RK4's aux only
+ * if a step was taken), so the arrays hold the reference's values between
+ * calls.  No counterpart in the
  * reference as an entry point (its loop is host Python). */
 enum {
   SLES_DRIVE_STATE_UMAX = 0,     /* max u of the last state examined */

@@ -22,6 +22,8 @@ struct sles_plan_s {
   int tune_fuse_x = 1;   // RK4 x-forward runs the next stage's x-inverse where compiled
   int tune_fft_wide = 0; // generic transforms: wide tiles in the x (bit 0) / y (bit 1) pass
                          // (default set per size in sles_plan_create)
+  int tune_pad = 0;      // fused-step workspace: kx planes padded by this many 128-byte lines
+                         // (default set per size in sles_plan_create)
 };
 
 namespace sles {
@@ -32,6 +34,17 @@ int check_launch(const char* what);
 
 inline long nreal(const sles_plan_s* p) { return (long)p->n * p->n * p->n; }
 inline long ncplx(const sles_plan_s* p) { return (long)p->n * p->n * p->nh; }
+inline long cplx_bytes(const sles_plan_s* p) { return p->precision == 0 ? 16 : 8; }  // one spectral value
+
+// Fused-step workspace layout: kx planes of N (N/2+1) spectral values start
+// every plane_w elements (padded by tune_pad 128-byte lines: unpadded planes
+// are a multiple of 4 KiB apart at N = 256 and 512, and the strided x passes,
+// which read 128-256-byte runs from every plane, then stream ~25 % slower),
+// fields every cplx_w = N plane_w elements.
+inline long plane_w(const sles_plan_s* p) {
+  return (long)p->n * p->nh + (long)p->tune_pad * 128 / cplx_bytes(p);
+}
+inline long cplx_w(const sles_plan_s* p) { return (long)p->n * plane_w(p); }
 
 // Wavenumbers of the reference layout (initialize_domain.py:76-79,99-107):
 // axes 0 and 1 use fftfreq(N, 1/N) (index N/2 -> -N/2); axis 2 holds 0..N/2
@@ -104,7 +117,9 @@ using WideShape = StridedShape<T, N, PL, (sizeof(T) == 4 && PL::NT >= 64) ? 1024
 // (sles_rhs.hip); sles_plan_create calls it with that device current.
 void step_scratch_init(int device);
 
-int z_c2r_rows(const sles_plan_s* p, const void* in, void* out, long nrows, double scale, hipStream_t s);
+// c2r along z of nrows half-spectrum rows (row r: plane r / N, row r % N of
+// planes `plane` elements apart) into nrows real rows of N.
+int z_c2r_rows(const sles_plan_s* p, const void* in, long plane, void* out, long nrows, double scale, hipStream_t s);
 
 }  // namespace sles
 

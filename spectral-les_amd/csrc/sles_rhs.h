@@ -118,13 +118,23 @@ struct Cols {
   }
 };
 
+// Rows t + m NT of column g in an x pass over an array whose kx planes lie
+// `plane` elements apart (N (N/2+1) for the reference layout; the workspace
+// and the driver's state copies use padded planes).
+struct XRows {
+  eidx base, step;
+  __device__ __forceinline__ XRows(int t, int nt, long plane, long g)
+      : base((eidx)t * plane + (eidx)g), step((eidx)nt * plane) {}
+  __device__ __forceinline__ eidx operator()(int m) const { return base + (eidx)m * step; }
+};
+
 // ---------------------------------------------------------------------------
 // 1. x-inverse
 // ---------------------------------------------------------------------------
 template <typename T, int N, int LES, class PL>
 __global__ void __launch_bounds__((WideShape<T, N, PL>::THREADS))
-    k_xinv(const cpx<T>* __restrict__ uh, cpx<T>* __restrict__ W, long C, const cpx<T>* __restrict__ tw, int xcd,
-           Members mb) {
+    k_xinv(const cpx<T>* __restrict__ uh, long ps, cpx<T>* __restrict__ W, long pw, const cpx<T>* __restrict__ tw,
+           int xcd, Members mb) {
   uh = member_ptr(uh, mb.state);
   W = member_ptr(W, mb.work);
   using S = WideShape<T, N, PL>;
@@ -136,9 +146,8 @@ __global__ void __launch_bounds__((WideShape<T, N, PL>::THREADS))
   const int t = threadIdx.x / TK;
   const TwRegs<T, PL> twr(tw, t);  // read once per block, used by every transform
   const LdsColumn<TK> map{col.c};
-  const T ky = (T)kfreq((int)(col.g / Cols<N, TK>::NH), N);
-  const T kz = (T)(col.g % Cols<N, TK>::NH);
-  auto at = [&](int m) { return (eidx)(t + m * NT) * (eidx)Cols<N, TK>::M + (eidx)col.g; };
+  const XRows rs(t, NT, ps, col.g), rw(t, NT, pw, col.g);  // U_hat rows, W rows
+  const long CS = N * ps, C = N * pw;
   cpx<T> v[P];
   if constexpr (LES) {
 #pragma unroll 1
@@ -146,17 +155,17 @@ __global__ void __launch_bounds__((WideShape<T, N, PL>::THREADS))
       cpx<T> w[P];
 #pragma unroll
       for (int m = 0; m < P; ++m) {
-        v[m] = ldx(uh + j * C, at(m));
+        v[m] = ldx(uh + j * CS, rs(m));
         w[m] = mul_i((T)kfreq(t + m * NT, N) * v[m]);  // 1j * K0 * U_j
       }
       FFT::run(v, lds, map, t, twr);
       if (col.valid)
 #pragma unroll
-        for (int m = 0; m < P; ++m) stx(W + j * C, at(m), v[m]);
+        for (int m = 0; m < P; ++m) stx(W + j * C, rw(m), v[m]);
       FFT::run(w, lds, map, t, twr);
       if (col.valid)
 #pragma unroll
-        for (int m = 0; m < P; ++m) stx(W + (3 + j) * C, at(m), w[m]);
+        for (int m = 0; m < P; ++m) stx(W + (3 + j) * C, rw(m), w[m]);
     }
   } else {
     // U_hat and the x-derivatives of its y and z components, from one read
@@ -166,16 +175,16 @@ __global__ void __launch_bounds__((WideShape<T, N, PL>::THREADS))
     cpx<T> u0[P], u1[P], u2[P];
 #pragma unroll
     for (int m = 0; m < P; ++m) {
-      u0[m] = ldx(uh, at(m));
-      u1[m] = ldx(uh + C, at(m));
-      u2[m] = ldx(uh + 2 * C, at(m));
+      u0[m] = ldx(uh, rs(m));
+      u1[m] = ldx(uh + CS, rs(m));
+      u2[m] = ldx(uh + 2 * CS, rs(m));
     }
     auto kx = [&](int m) { return (T)kfreq(t + m * NT, N); };
     auto fft_store = [&](cpx<T>(&x)[P], int j) {
       FFT::run(x, lds, map, t, twr);
       if (col.valid)
 #pragma unroll
-        for (int m = 0; m < P; ++m) stx(W + j * C, at(m), x[m]);
+        for (int m = 0; m < P; ++m) stx(W + j * C, rw(m), x[m]);
     };
 #pragma unroll
     for (int m = 0; m < P; ++m) v[m] = mul_i(kx(m) * u1[m]);
@@ -194,7 +203,7 @@ __global__ void __launch_bounds__((WideShape<T, N, PL>::THREADS))
 // ---------------------------------------------------------------------------
 template <typename T, int N, int LES, class PL>
 __global__ void __launch_bounds__((WideShape<T, N, PL>::THREADS))
-    k_yinv(cpx<T>* W, long C, const cpx<T>* __restrict__ tw, int xcd, Members mb) {
+    k_yinv(cpx<T>* W, long pw, const cpx<T>* __restrict__ tw, int xcd, Members mb) {
   W = member_ptr(W, mb.work);
   using S = WideShape<T, N, PL>;
   constexpr int P = S::P, NT = S::NT, TK = S::TK, NH = N / 2 + 1;
@@ -206,7 +215,9 @@ __global__ void __launch_bounds__((WideShape<T, N, PL>::THREADS))
   const TwRegs<T, PL> twr(tw, t);  // read once per block, used by every transform
   const LdsColumn<TK> map{col.c};
   const long ix = col.g / NH, iz = col.g % NH;
-  auto at = [&](int m) { return (eidx)ix * (eidx)(N * NH) + (eidx)(t + m * NT) * NH + (eidx)iz; };
+  const eidx row0 = (eidx)ix * pw + (eidx)t * NH + (eidx)iz;
+  auto at = [&](int m) { return row0 + (eidx)(m * NT * NH); };
+  const long C = N * pw;
   cpx<T> v[P];
   if constexpr (LES) {
 #pragma unroll 1
@@ -390,7 +401,7 @@ __device__ __forceinline__ void block_max_store(double v, double* out) {
 
 template <typename T, int N, class PL, int MODEL>
 __global__ void __launch_bounds__(ZLines<PL>::THREADS)
-    k_zles(cpx<T>* W, long C, T scale, T coeff, double* umax_part, const cpx<T>* __restrict__ tw, Members mb) {
+    k_zles(cpx<T>* W, long pw, T scale, T coeff, double* umax_part, const cpx<T>* __restrict__ tw, Members mb) {
   W = member_ptr(W, mb.work);
   if (umax_part) umax_part = member_ptr(umax_part, mb.work);
   if (mb.coeff) coeff = (T)mb.coeff[blockIdx.y];
@@ -403,7 +414,9 @@ __global__ void __launch_bounds__(ZLines<PL>::THREADS)
   const int line = threadIdx.x / NT, t = threadIdx.x % NT;
   const long row = (long)blockIdx.x * LPB + line;
   const bool valid = row < (long)N * N;
-  const unsigned row0 = (unsigned)((valid ? row : 0) * NH);
+  const long rv = valid ? row : 0;  // (x, y) line: plane x = rv / N, row y = rv % N
+  const unsigned row0 = (unsigned)((rv / N) * pw + (rv % N) * NH);
+  const long C = N * pw;
   const LdsLine<T, N> map{line * LdsLine<T, N>::STRIDE};
   const TwRegs<T, PL> twr(tw, t);  // read once, used by all 11 transforms
   cpx<T>* w[9];
@@ -521,7 +534,7 @@ __global__ void __launch_bounds__(ZLines<PL>::THREADS)
 
 template <typename T, int N, class PL>
 __global__ void __launch_bounds__(ZLines<PL>::THREADS)
-    k_zns(cpx<T>* W, long C, T scale, double* umax_part, const cpx<T>* __restrict__ tw, Members mb) {
+    k_zns(cpx<T>* W, long pw, T scale, double* umax_part, const cpx<T>* __restrict__ tw, Members mb) {
   W = member_ptr(W, mb.work);
   if (umax_part) umax_part = member_ptr(umax_part, mb.work);
   constexpr int P = PL::P, NT = PL::NT, NH = N / 2 + 1, KH = P / 2 + 1;
@@ -533,7 +546,9 @@ __global__ void __launch_bounds__(ZLines<PL>::THREADS)
   const int line = threadIdx.x / NT, t = threadIdx.x % NT;
   const long row = (long)blockIdx.x * LPB + line;
   const bool valid = row < (long)N * N;
-  const unsigned row0 = (unsigned)((valid ? row : 0) * NH);
+  const long rv = valid ? row : 0;  // (x, y) line: plane x = rv / N, row y = rv % N
+  const unsigned row0 = (unsigned)((rv / N) * pw + (rv % N) * NH);
+  const long C = N * pw;
   const LdsLine<T, N> map{line * LdsLine<T, N>::STRIDE};
   const TwRegs<T, PL> twr(tw, t);  // read once, used by all 5 transforms
   cpx<T>* w[6];
@@ -591,7 +606,7 @@ __global__ void __launch_bounds__(ZLines<PL>::THREADS)
 // ---------------------------------------------------------------------------
 template <typename T, int N, int LES, class PL>
 __global__ void __launch_bounds__((WideShape<T, N, PL>::THREADS))
-    k_yfwd(cpx<T>* W, long C, const cpx<T>* __restrict__ tw, int xcd, Members mb) {
+    k_yfwd(cpx<T>* W, long pw, const cpx<T>* __restrict__ tw, int xcd, Members mb) {
   W = member_ptr(W, mb.work);
   using S = WideShape<T, N, PL>;
   constexpr int P = S::P, NT = S::NT, TK = S::TK, NH = N / 2 + 1;
@@ -605,7 +620,9 @@ __global__ void __launch_bounds__((WideShape<T, N, PL>::THREADS))
   const TwRegs<T, PL> twr(tw, t);  // read once per block, used by every transform
   const LdsColumn<TK> map{col.c};
   const long ix = col.g / NH, iz = col.g % NH;
-  auto at = [&](int m) { return (eidx)ix * (eidx)(N * NH) + (eidx)(t + m * NT) * NH + (eidx)iz; };
+  const eidx row0 = (eidx)ix * pw + (eidx)t * NH + (eidx)iz;
+  auto at = [&](int m) { return row0 + (eidx)(m * NT * NH); };
+  const long C = N * pw;
   cpx<T> v[P], acc[P];
   auto load = [&](int f) {
 #pragma unroll
@@ -670,7 +687,8 @@ __device__ __forceinline__ double phi2_d(double z) {
   return (expm1(z) - z) / (z * z);
 }
 
-// ynext (RK4 stages 0-2): also receives the next stage's input U_hat.
+// ynext (RK4 stages 0-2): also receives the next stage's input U_hat.  e and
+// C: element and field stride of the RK-state arrays (and of a.stages).
 template <typename T>
 __device__ __forceinline__ void epilogue(cpx<T> (&s)[3], eidx e, int kx, int ky, int kz, int n, const StepArgs& a,
                                          double dt, cpx<T>* uh, cpx<T>* uh0, cpx<T>* uh1, cpx<T>* aux, long C,
@@ -794,7 +812,7 @@ using XShape = StridedShape<T, N, PL, (sizeof(T) == 4 && PL::P == 4) ? 1024 : 51
 // stage runs that pass's x-inverse on the new U_hat).
 template <typename T, int N, int LES, class PL, bool FUSE = false, int FLES = LES>
 __global__ void __launch_bounds__((XShape<T, N, PL>::THREADS))
-    k_xfwd(cpx<T>* __restrict__ W, long C, cpx<T>* uh, cpx<T>* uh0, cpx<T>* uh1, cpx<T>* aux, StepArgs a,
+    k_xfwd(cpx<T>* __restrict__ W, long pw, cpx<T>* uh, cpx<T>* uh0, cpx<T>* uh1, cpx<T>* aux, long ps, StepArgs a,
            const cpx<T>* __restrict__ tw, int xcd, Members mb) {
   W = member_ptr(W, mb.work);
   uh = member_ptr(uh, mb.state);
@@ -820,7 +838,8 @@ __global__ void __launch_bounds__((XShape<T, N, PL>::THREADS))
   const int t = threadIdx.x / TK;
   const TwRegs<T, PL> twr(tw, t);  // read once per block, used by every transform
   const LdsColumn<TK> map{col.c};
-  auto at = [&](int m) { return (eidx)(t + m * NT) * (eidx)Cols<N, TK>::M + (eidx)col.g; };
+  const XRows rw(t, NT, pw, col.g), rs(t, NT, ps, col.g);  // W rows, RK-state rows
+  const long C = N * pw, CS = N * ps;
   // a0..a2: S_i = F_x(W_i) (+ 2i Kx F_x(tau_xi) with LES).  Named arrays and
   // a rolled field loop with uniform branches keep them in registers.
   cpx<T> a0[P], a1[P], a2[P], v[P];
@@ -829,14 +848,14 @@ __global__ void __launch_bounds__((XShape<T, N, PL>::THREADS))
   const int nf = skip ? 0 : NF;
   cpx<T> nx[P];
 #pragma unroll
-  for (int m = 0; m < P; ++m) nx[m] = ldx(W, at(m));
+  for (int m = 0; m < P; ++m) nx[m] = ldx(W, rw(m));
 #pragma unroll 1
   for (int f = 0; f < nf; ++f) {
 #pragma unroll
     for (int m = 0; m < P; ++m) v[m] = nx[m];
     if (f + 1 < NF) {
 #pragma unroll
-      for (int m = 0; m < P; ++m) nx[m] = ldx(W + (f + 1) * C, at(m));
+      for (int m = 0; m < P; ++m) nx[m] = ldx(W + (f + 1) * C, rw(m));
     }
     FFT::run(v, lds, map, t, twr);
     if (f < 3) {
@@ -872,10 +891,10 @@ __global__ void __launch_bounds__((XShape<T, N, PL>::THREADS))
     cpx<T> s3[3] = {a0[m], a1[m], a2[m]};
     cpx<T> yn[3] = {{T(0), T(0)}, {T(0), T(0)}, {T(0), T(0)}};
     if (col.valid && !skip) {
-      epilogue<T>(s3, at(m), kfreq(t + m * NT, N), ky, kz, N, a, dt, uh, uh0, uh1, aux, C, yn);
+      epilogue<T>(s3, rs(m), kfreq(t + m * NT, N), ky, kz, N, a, dt, uh, uh0, uh1, aux, CS, yn);
     } else if (col.valid) {
 #pragma unroll
-      for (int i = 0; i < 3; ++i) yn[i] = ldx(uh + i * C, at(m));
+      for (int i = 0; i < 3; ++i) yn[i] = ldx(uh + i * CS, rs(m));
     }
     y0[m] = yn[0];
     y1[m] = yn[1];
@@ -888,7 +907,7 @@ __global__ void __launch_bounds__((XShape<T, N, PL>::THREADS))
       IFFT::run(x, lds, map, t, twr);
       if (col.valid)
 #pragma unroll
-        for (int m = 0; m < P; ++m) stx(W + j * C, at(m), x[m]);
+        for (int m = 0; m < P; ++m) stx(W + j * C, rw(m), x[m]);
     };
     if constexpr (FLES) {
       // as k_xinv: W_j = F_x^-1 Y_j, W_{3+j} = F_x^-1 (1j Kx Y_j)
@@ -1083,10 +1102,10 @@ struct Fused {
   static constexpr bool FUSE_X = sizeof(T) == 4 || N <= 64;
 
   // skip_xinv: W0-5 already hold the x-inverse output (fused into the previous x-forward)
-  // nm members (grid rows) at the strides of mb
-  static int stage(const sles_plan_s* p, int les, T coeff, cpx<T>* uh, cpx<T>* W, double* part, hipStream_t s,
-                   bool skip_xinv, const Members& mb, int nm) {
-    const long C = ncplx(p);
+  // nm members (grid rows) at the strides of mb; ps: kx-plane stride of the RK-state arrays
+  static int stage(const sles_plan_s* p, int les, T coeff, cpx<T>* uh, long ps, cpx<T>* W, double* part,
+                   hipStream_t s, bool skip_xinv, const Members& mb, int nm) {
+    const long pw = plane_w(p);
     const cpx<T>* tw = (const cpx<T>*)p->tw;
     const int xcd = p->tune_xcd;
     constexpr long NCOL = (long)N * (N / 2 + 1);  // columns of every strided pass
@@ -1096,86 +1115,92 @@ struct Fused {
     if (les) {
       if (!skip_xinv) {
         ProfScope ps_(PK_XINV, s);
-        hipLaunchKernelGGL((k_xinv<T, N, 1, PS>), gs, bs, 0, s, uh, W, C, tw, xcd, mb);
+        hipLaunchKernelGGL((k_xinv<T, N, 1, PS>), gs, bs, 0, s, uh, ps, W, pw, tw, xcd, mb);
       }
-      { ProfScope ps_(PK_YINV, s); hipLaunchKernelGGL((k_yinv<T, N, 1, PS>), gs, bs, 0, s, W, C, tw, xcd, mb); }
+      { ProfScope ps_(PK_YINV, s); hipLaunchKernelGGL((k_yinv<T, N, 1, PS>), gs, bs, 0, s, W, pw, tw, xcd, mb); }
       ProfScope ps_(PK_Z, s);
       if (les == SLES_LES_SMAGORINSKY)
-        hipLaunchKernelGGL((k_zles<T, N, PZ, SLES_LES_SMAGORINSKY>), gz, bz, 0, s, W, C, scale, coeff, part, tw, mb);
+        hipLaunchKernelGGL((k_zles<T, N, PZ, SLES_LES_SMAGORINSKY>), gz, bz, 0, s, W, pw, scale, coeff, part, tw, mb);
       else
-        hipLaunchKernelGGL((k_zles<T, N, PZ, SLES_LES_VREMAN>), gz, bz, 0, s, W, C, scale, coeff, part, tw, mb);
+        hipLaunchKernelGGL((k_zles<T, N, PZ, SLES_LES_VREMAN>), gz, bz, 0, s, W, pw, scale, coeff, part, tw, mb);
     } else {
       if (!skip_xinv) {
         ProfScope ps_(PK_XINV, s);
-        hipLaunchKernelGGL((k_xinv<T, N, 0, PS>), gs, bs, 0, s, uh, W, C, tw, xcd, mb);
+        hipLaunchKernelGGL((k_xinv<T, N, 0, PS>), gs, bs, 0, s, uh, ps, W, pw, tw, xcd, mb);
       }
-      { ProfScope ps_(PK_YINV, s); hipLaunchKernelGGL((k_yinv<T, N, 0, PS>), gs, bs, 0, s, W, C, tw, xcd, mb); }
+      { ProfScope ps_(PK_YINV, s); hipLaunchKernelGGL((k_yinv<T, N, 0, PS>), gs, bs, 0, s, W, pw, tw, xcd, mb); }
       ProfScope ps_(PK_Z, s);
-      hipLaunchKernelGGL((k_zns<T, N, PZ>), gz, bz, 0, s, W, C, scale, part, tw, mb);
+      hipLaunchKernelGGL((k_zns<T, N, PZ>), gz, bz, 0, s, W, pw, scale, part, tw, mb);
     }
     return check_launch("fused stage (inverse)") ? -1 : 0;
   }
 
   // fuse_next: also run the next stage's x-inverse (RK4 stages 0-2)
   // fuse_phys (last RK4 stage): also run the physical-output pass's x-inverse
-  static int finish(const sles_plan_s* p, int les, cpx<T>* uh, cpx<T>* uh0, cpx<T>* uh1, cpx<T>* aux, cpx<T>* W,
-                    const StepArgs& a, hipStream_t s, bool fuse_next, bool fuse_phys, const Members& mb, int nm) {
-    const long C = ncplx(p);
+  static int finish(const sles_plan_s* p, int les, cpx<T>* uh, cpx<T>* uh0, cpx<T>* uh1, cpx<T>* aux, long ps,
+                    cpx<T>* W, const StepArgs& a, hipStream_t s, bool fuse_next, bool fuse_phys, const Members& mb,
+                    int nm) {
+    const long pw = plane_w(p);
     const cpx<T>* tw = (const cpx<T>*)p->tw;
     const int xcd = p->tune_xcd;
     constexpr long NCOL = (long)N * (N / 2 + 1);  // columns of every strided pass
     const dim3 gs((unsigned)((NCOL + SS::TK - 1) / SS::TK), (unsigned)nm), bs(SS::THREADS);
     const dim3 gx((unsigned)((NCOL + SX::TK - 1) / SX::TK), (unsigned)nm), bx(SX::THREADS);
     if (les) {
-      { ProfScope ps_(PK_YFWD, s); hipLaunchKernelGGL((k_yfwd<T, N, 1, PS>), gs, bs, 0, s, W, C, tw, xcd, mb); }
+      { ProfScope ps_(PK_YFWD, s); hipLaunchKernelGGL((k_yfwd<T, N, 1, PS>), gs, bs, 0, s, W, pw, tw, xcd, mb); }
       ProfScope ps_(PK_XFWD, s);
       if constexpr (FUSE_X) {
         if (fuse_next) {
-          hipLaunchKernelGGL((k_xfwd<T, N, 1, PX, true>), gx, bx, 0, s, W, C, uh, uh0, uh1, aux, a, tw, xcd, mb);
+          hipLaunchKernelGGL((k_xfwd<T, N, 1, PX, true>), gx, bx, 0, s, W, pw, uh, uh0, uh1, aux, ps, a, tw, xcd, mb);
           return check_launch("fused stage (forward)");
         }
         if (fuse_phys) {
-          hipLaunchKernelGGL((k_xfwd<T, N, 1, PX, true, 0>), gx, bx, 0, s, W, C, uh, uh0, uh1, aux, a, tw, xcd, mb);
+          hipLaunchKernelGGL((k_xfwd<T, N, 1, PX, true, 0>), gx, bx, 0, s, W, pw, uh, uh0, uh1, aux, ps, a, tw, xcd,
+                             mb);
           return check_launch("fused stage (forward)");
         }
       }
-      hipLaunchKernelGGL((k_xfwd<T, N, 1, PX>), gx, bx, 0, s, W, C, uh, uh0, uh1, aux, a, tw, xcd, mb);
+      hipLaunchKernelGGL((k_xfwd<T, N, 1, PX>), gx, bx, 0, s, W, pw, uh, uh0, uh1, aux, ps, a, tw, xcd, mb);
     } else {
-      { ProfScope ps_(PK_YFWD, s); hipLaunchKernelGGL((k_yfwd<T, N, 0, PS>), gs, bs, 0, s, W, C, tw, xcd, mb); }
+      { ProfScope ps_(PK_YFWD, s); hipLaunchKernelGGL((k_yfwd<T, N, 0, PS>), gs, bs, 0, s, W, pw, tw, xcd, mb); }
       ProfScope ps_(PK_XFWD, s);
       if constexpr (FUSE_X)
         if (fuse_next || fuse_phys) {  // the same output layout
-          hipLaunchKernelGGL((k_xfwd<T, N, 0, PX, true>), gx, bx, 0, s, W, C, uh, uh0, uh1, aux, a, tw, xcd, mb);
+          hipLaunchKernelGGL((k_xfwd<T, N, 0, PX, true>), gx, bx, 0, s, W, pw, uh, uh0, uh1, aux, ps, a, tw, xcd, mb);
           return check_launch("fused stage (forward)");
         }
-      hipLaunchKernelGGL((k_xfwd<T, N, 0, PX>), gx, bx, 0, s, W, C, uh, uh0, uh1, aux, a, tw, xcd, mb);
+      hipLaunchKernelGGL((k_xfwd<T, N, 0, PX>), gx, bx, 0, s, W, pw, uh, uh0, uh1, aux, ps, a, tw, xcd, mb);
     }
     return check_launch("fused stage (forward)");
   }
 
   // skip_xinv: W0-4 already hold the x-inverse output (fused into the last x-forward)
-  static int physical(const sles_plan_s* p, cpx<T>* uh, cpx<T>* W, void* u_out, void* curl_out, hipStream_t s,
-                      bool skip_xinv = false) {
-    const long C = ncplx(p);
+  static int physical(const sles_plan_s* p, cpx<T>* uh, long ps, cpx<T>* W, void* u_out, void* curl_out,
+                      hipStream_t s, bool skip_xinv = false) {
+    const long pw = plane_w(p);
     const cpx<T>* tw = (const cpx<T>*)p->tw;
     const int xcd = p->tune_xcd;
     constexpr long NCOL = (long)N * (N / 2 + 1);  // columns of every strided pass
     const dim3 gs((unsigned)((NCOL + SS::TK - 1) / SS::TK)), bs(SS::THREADS);
     ProfScope ps_(PK_PHYS, s);
-    if (!skip_xinv) hipLaunchKernelGGL((k_xinv<T, N, 0, PS>), gs, bs, 0, s, uh, W, C, tw, xcd, Members{});
-    hipLaunchKernelGGL((k_yinv<T, N, 0, PS>), gs, bs, 0, s, W, C, tw, xcd, Members{});
+    if (!skip_xinv) hipLaunchKernelGGL((k_xinv<T, N, 0, PS>), gs, bs, 0, s, uh, ps, W, pw, tw, xcd, Members{});
+    hipLaunchKernelGGL((k_yinv<T, N, 0, PS>), gs, bs, 0, s, W, pw, tw, xcd, Members{});
     if (check_launch("fused physical")) return -1;
     const double sc = 1.0 / ((double)N * N * N);
-    if (u_out && z_c2r_rows(p, W, u_out, 3L * N * N, sc, s)) return -1;
-    if (curl_out && z_c2r_rows(p, W + 3 * C, curl_out, 3L * N * N, sc, s)) return -1;
+    if (u_out && z_c2r_rows(p, W, pw, u_out, 3L * N * N, sc, s)) return -1;
+    if (curl_out && z_c2r_rows(p, W + 3 * N * pw, pw, curl_out, 3L * N * N, sc, s)) return -1;
     return 0;
   }
 };
 
+// One fused step (or RHS) of `scheme` over the RK-state arrays uh, uh0, uh1,
+// aux, whose kx planes lie ps elements apart (the reference layout:
+// N (N/2+1); the driver's padded copies: plane_w); the workspace W holds
+// the padded fields and the CFL partials (work_fields_bytes).
 template <typename T, int N, class PS, class PX, class PZ>
 int step_impl(const sles_plan_s* p, int scheme, const sles_step_params* prm, void* uh_, void* uh0_, void* uh1_,
-              void* aux_, void* u_out, void* curl_out, double* scalars, void* work, hipStream_t s, void* stages,
-              const DriveArgs* drive, const Batch* batch) {
+              void* aux_, long ps, void* u_out, void* curl_out, double* scalars, void* work, hipStream_t s,
+              void* stages, const DriveArgs* drive, const Batch* batch) {
   using F = Fused<T, N, PS, PX, PZ>;
   const int nm = batch ? batch->count : 1;
   const Members mb = batch ? batch->mb : Members{0, 0, 0, nullptr};
@@ -1184,10 +1209,9 @@ int step_impl(const sles_plan_s* p, int scheme, const sles_step_params* prm, voi
   cpx<T>* uh1 = (cpx<T>*)uh1_;
   cpx<T>* aux = (cpx<T>*)aux_;
   cpx<T>* W = (cpx<T>*)work;
-  const long C = ncplx(p);
   const int les = prm->les_model;
   const int nw = les ? 9 : 6;
-  double* part = (double*)(W + nw * C);
+  double* part = (double*)(W + nw * cplx_w(p));
   const bool ddt = prm->dt_device != 0 || drive;
   StepArgs a{};
   a.dt = prm->dt;
@@ -1216,7 +1240,7 @@ int step_impl(const sles_plan_s* p, int scheme, const sles_step_params* prm, voi
     double* pp = (ddt && st == 0) ? part : nullptr;
     const bool fuse = F::FUSE_X && p->tune_fuse_x && scheme == SLES_SCHEME_RK4;
     const bool fused_in = fuse && st > 0;  // x-inverse ran in the previous x-forward
-    if (F::stage(p, les, coeff, uh, W, pp, s, fused_in, mb, nm)) return -1;
+    if (F::stage(p, les, coeff, uh, ps, W, pp, s, fused_in, mb, nm)) return -1;
     if (pp) {
       // two levels: up to 64 chunks of the per-block maxima, then one block
       // (the partials area holds N^2 + 64 doubles; the z grid is at most N^2)
@@ -1234,13 +1258,13 @@ int step_impl(const sles_plan_s* p, int scheme, const sles_step_params* prm, voi
                            scalars);
       if (check_launch("cfl")) return -1;
     }
-    if (F::finish(p, les, uh, uh0, uh1, aux, W, a, s, fuse && st < 3, fuse && st == 3 && (u_out || curl_out), mb,
-                  nm))
+    if (F::finish(p, les, uh, uh0, uh1, aux, ps, W, a, s, fuse && st < 3, fuse && st == 3 && (u_out || curl_out),
+                  mb, nm))
       return -1;
   }
   // the last fused RK4 x-forward already ran the physical pass's x-inverse
   const bool phys_fused = F::FUSE_X && p->tune_fuse_x && scheme == SLES_SCHEME_RK4;
-  if (u_out || curl_out) return F::physical(p, uh, W, u_out, curl_out, s, phys_fused);
+  if (u_out || curl_out) return F::physical(p, uh, ps, W, u_out, curl_out, s, phys_fused);
   return 0;
 }
 
@@ -1256,7 +1280,7 @@ struct HasP4 {
 
 template <typename T, int N>
 int step_t(const sles_plan_s* p, int scheme, const sles_step_params* prm, void* uh, void* uh0, void* uh1, void* aux,
-           void* u_out, void* curl_out, double* scalars, void* work, hipStream_t s, void* stages,
+           long ps, void* u_out, void* curl_out, double* scalars, void* work, hipStream_t s, void* stages,
            const DriveArgs* drive, const Batch* batch) {
   using D = LinePlan<N>;
   if constexpr (HasP4<N>::value) {
@@ -1264,8 +1288,8 @@ int step_t(const sles_plan_s* p, int scheme, const sles_step_params* prm, void* 
     const int sel = (p->tune_strided ? 1 : 0) | (p->tune_xfwd ? 2 : 0) | (p->tune_z ? 4 : 0);
 #define SLES_SEL(K, A, B, Cz) \
   case K:                     \
-    return step_impl<T, N, A, B, Cz>(p, scheme, prm, uh, uh0, uh1, aux, u_out, curl_out, scalars, work, s, stages, \
-                                     drive, batch);
+    return step_impl<T, N, A, B, Cz>(p, scheme, prm, uh, uh0, uh1, aux, ps, u_out, curl_out, scalars, work, s, \
+                                     stages, drive, batch);
     switch (sel) {
       SLES_SEL(0, D, D, D)
       SLES_SEL(1, Q, D, D)
@@ -1278,16 +1302,17 @@ int step_t(const sles_plan_s* p, int scheme, const sles_step_params* prm, void* 
     }
 #undef SLES_SEL
   }
-  return step_impl<T, N, D, D, D>(p, scheme, prm, uh, uh0, uh1, aux, u_out, curl_out, scalars, work, s, stages,
-                                  drive, batch);
+  return step_impl<T, N, D, D, D>(p, scheme, prm, uh, uh0, uh1, aux, ps, u_out, curl_out, scalars, work, s,
+                                  stages, drive, batch);
 }
 
 // Per-precision instantiation (sles_rhs_f64.hip / sles_rhs_f32.hip).
+// ps: kx-plane stride (elements) of the RK-state arrays uh, uh0, uh1, aux, stages.
 int step_f64(const sles_plan_s* p, int scheme, const sles_step_params* prm, void* uh, void* uh0, void* uh1,
-             void* aux, void* u_out, void* curl_out, double* scalars, void* work, hipStream_t s, void* stages,
-             const DriveArgs* drive = nullptr, const Batch* batch = nullptr);
+             void* aux, long ps, void* u_out, void* curl_out, double* scalars, void* work, hipStream_t s,
+             void* stages, const DriveArgs* drive = nullptr, const Batch* batch = nullptr);
 int step_f32(const sles_plan_s* p, int scheme, const sles_step_params* prm, void* uh, void* uh0, void* uh1,
-             void* aux, void* u_out, void* curl_out, double* scalars, void* work, hipStream_t s, void* stages,
-             const DriveArgs* drive = nullptr, const Batch* batch = nullptr);
+             void* aux, long ps, void* u_out, void* curl_out, double* scalars, void* work, hipStream_t s,
+             void* stages, const DriveArgs* drive = nullptr, const Batch* batch = nullptr);
 
 }  // namespace sles

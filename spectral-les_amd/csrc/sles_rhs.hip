@@ -53,15 +53,142 @@ static void* step_scratch(int device, std::string* why) {
 }
 }  // namespace sles
 
+namespace sles {
+namespace {
+
+// Workspace of the fused step: the W fields (planes padded by tune_pad) and
+// the CFL partials; with padded planes also the drivers' copies of the RK
+// state in the same layout (U_hat, U_hat0, U_hat1, aux: 12 fields, at
+// mirror_offset), preceded by one 256-byte slot per member for the steps
+// count at the start of a call.
+size_t work_fields_bytes(const sles_plan_s* p, int les_model) {
+  const size_t nw = les_model ? 9 : 6;
+  return nw * (size_t)cplx_w(p) * (size_t)cplx_bytes(p) + sizeof(double) * ((size_t)p->n * p->n + 64);
+}
+size_t steps_slot_offset(const sles_plan_s* p, int les_model) {
+  return (work_fields_bytes(p, les_model) + 255) / 256 * 256;
+}
+size_t mirror_offset(const sles_plan_s* p, int les_model) { return steps_slot_offset(p, les_model) + 256; }
+constexpr int kMirrorFields = 12;
+
+// Copies 3 spectral fields of N kx planes (plane_bytes each) from src to
+// dst, planes sp / dp bytes apart (fields N planes apart); blockIdx.z is the
+// member (sm / dm bytes apart).  With `steps` set, member m copies only if
+// its driver state counts a step taken since `before` saved the count (so a
+// call that took no step leaves the destination as it was).
+__global__ void k_relayout(const char* __restrict__ src, long sp, long sm, char* __restrict__ dst, long dp, long dm,
+                           long plane_bytes, int n, const double* steps, const double* before, long drive_stride,
+                           long before_stride) {
+  const long m = blockIdx.z;
+  if (steps) {
+    const double now = *(const double*)((const char*)steps + m * drive_stride);
+    const double then = *(const double*)((const char*)before + m * before_stride);
+    if (now == then) return;
+  }
+  const long plane = blockIdx.y;  // field * n + kx plane
+  const char* s = src + m * sm + plane * sp;
+  char* d = dst + m * dm + plane * dp;
+  const long stride = (long)gridDim.x * blockDim.x * 16;
+  for (long o = ((long)blockIdx.x * blockDim.x + threadIdx.x) * 16; o < plane_bytes; o += stride)
+    *(uint4*)(d + o) = *(const uint4*)(s + o);
+}
+
+// before[m] = the steps count of member m's driver state, m < members
+__global__ void k_save_steps(const double* steps, long drive_stride, double* before, long before_stride,
+                             int members) {
+  const long m = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m < members)
+    *(double*)((char*)before + m * before_stride) = *(const double*)((const char*)steps + m * drive_stride);
+}
+
+int relayout(const sles_plan_s* p, const void* src, long sp, long sm, void* dst, long dp, long dm, int members,
+             const double* steps, const double* before, long drive_stride, long before_stride, hipStream_t s) {
+  const long plane_bytes = ncplx(p) / p->n * cplx_bytes(p);
+  const unsigned chunks = (unsigned)((plane_bytes / 16 + 1023) / 1024);  // 256 threads x 4 copies each
+  hipLaunchKernelGGL(k_relayout, dim3(chunks, (unsigned)(3 * p->n), (unsigned)members), dim3(256), 0, s,
+                     (const char*)src, sp, sm, (char*)dst, dp, dm, plane_bytes, p->n, steps, before, drive_stride,
+                     before_stride);
+  return check_launch("state relayout");
+}
+
+// The RK-state arrays a driver call works on: the caller's (reference
+// layout) or, with padded workspace planes, copies in the workspace's layout
+// that the call fills from the caller's arrays on entry and copies back on
+// exit.
+struct DriveState {
+  void *uh, *uh0, *uh1, *aux;
+  long ps;          // kx-plane stride (elements)
+  long mstride;     // bytes between members' arrays
+};
+
+// Driver call entry: with padded workspace planes, point the call at the
+// state copies in each member's workspace (members `wstride` bytes apart)
+// and fill them from the caller's arrays (members `sbytes` apart): U_hat,
+// and aux for ETD2 (which reads F_{n-1}); U_hat0 / U_hat1 / aux of the RK
+// schemes are written by every step before they are read.  Also saves each
+// member's steps count, which drive_exit compares.
+int drive_enter(const sles_plan_s* p, int scheme, int les, void* uh, void* uh0, void* uh1, void* aux, int members,
+                long sbytes, double* state, long dstride, void* work, long wstride, hipStream_t s, DriveState* ds) {
+  const long dense = ncplx(p) / p->n;
+  if (p->tune_pad == 0) {
+    *ds = DriveState{uh, uh0, uh1, aux, dense, sbytes};
+    return 0;
+  }
+  // copies of the arrays the scheme touches: U_hat0 (RK4, RK3), U_hat1 (RK4), aux
+  const long fbytes = cplx_w(p) * cplx_bytes(p);
+  char* m = (char*)work + mirror_offset(p, les);
+  const bool rk = scheme == SLES_SCHEME_RK4 || scheme == SLES_SCHEME_RK3;
+  const bool rk4 = scheme == SLES_SCHEME_RK4;
+  *ds = DriveState{m, rk && uh0 ? m + 3 * fbytes : nullptr, rk4 && uh1 ? m + 6 * fbytes : nullptr,
+                   aux ? m + 9 * fbytes : nullptr, plane_w(p), wstride};
+  const long pb = dense * cplx_bytes(p), pwb = plane_w(p) * cplx_bytes(p);
+  double* before = (double*)((char*)work + steps_slot_offset(p, les));
+  hipLaunchKernelGGL(k_save_steps, dim3((unsigned)((members + 255) / 256)), dim3(256), 0, s,
+                     state + SLES_DRIVE_STATE_STEPS, dstride, before, wstride, members);
+  if (check_launch("driver steps count")) return -1;
+  if (relayout(p, uh, pb, sbytes, ds->uh, pwb, wstride, members, nullptr, nullptr, 0, 0, s)) return -1;
+  if (scheme == SLES_SCHEME_ETD2 && relayout(p, aux, pb, sbytes, ds->aux, pwb, wstride, members, nullptr, nullptr,
+                                             0, 0, s))
+    return -1;
+  return 0;
+}
+
+// Driver call exit: copy the state copies back to the caller's arrays --
+// U_hat always, and ETD2's aux; U_hat0, U_hat1 and RK4's aux (those the
+// scheme writes) only for members that took a step in this call (else the
+// copies hold nothing of it).
+int drive_exit(const sles_plan_s* p, int scheme, int les, void* uh, void* uh0, void* uh1, void* aux, int members,
+               long sbytes, double* state, long dstride, void* work, long wstride, hipStream_t s,
+               const DriveState& ds) {
+  if (p->tune_pad == 0) return 0;
+  const long pb = ncplx(p) / p->n * cplx_bytes(p), pwb = plane_w(p) * cplx_bytes(p);
+  const double* steps = state + SLES_DRIVE_STATE_STEPS;
+  const double* before = (const double*)((char*)work + steps_slot_offset(p, les));
+  if (relayout(p, ds.uh, pwb, wstride, uh, pb, sbytes, members, nullptr, nullptr, 0, 0, s)) return -1;
+  if (ds.uh0 && relayout(p, ds.uh0, pwb, wstride, uh0, pb, sbytes, members, steps, before, dstride, wstride, s))
+    return -1;
+  if (ds.uh1 && relayout(p, ds.uh1, pwb, wstride, uh1, pb, sbytes, members, steps, before, dstride, wstride, s))
+    return -1;
+  if (ds.aux) {
+    const bool always = scheme == SLES_SCHEME_ETD2;
+    if (relayout(p, ds.aux, pwb, wstride, aux, pb, sbytes, members, always ? nullptr : steps,
+                 always ? nullptr : before, dstride, wstride, s))
+      return -1;
+  }
+  return 0;
+}
+
+}  // namespace
+}  // namespace sles
+
 using namespace sles;
 
 extern "C" {
 
 size_t sles_step_work_bytes(sles_plan p, int les_model) {
   if (!p) return 0;
-  const size_t esz = p->precision == SLES_FP64 ? 16 : 8;
-  const size_t nw = les_model ? 9 : 6;
-  return nw * (size_t)ncplx(p) * esz + sizeof(double) * ((size_t)p->n * p->n + 64);
+  if (p->tune_pad == 0) return work_fields_bytes(p, les_model);
+  return mirror_offset(p, les_model) + (size_t)kMirrorFields * (size_t)cplx_w(p) * (size_t)cplx_bytes(p);
 }
 
 int sles_step(sles_plan p, int scheme, const sles_step_params* prm, void* u_hat, void* u_hat0, void* u_hat1,
@@ -78,9 +205,10 @@ int sles_step(sles_plan p, int scheme, const sles_step_params* prm, void* u_hat,
     return fail("sles_step: dt_device only applies to time steps");
   hipStream_t s = (hipStream_t)stream;
   if (!sles_supported_n(p->n)) return fail("sles_step: unsupported grid size");
+  const long ps = ncplx(p) / p->n;  // the caller's arrays: reference layout
   return p->precision == SLES_FP64
-             ? step_f64(p, scheme, prm, u_hat, u_hat0, u_hat1, aux, u_out, curl_out, scalars, work, s, nullptr)
-             : step_f32(p, scheme, prm, u_hat, u_hat0, u_hat1, aux, u_out, curl_out, scalars, work, s, nullptr);
+             ? step_f64(p, scheme, prm, u_hat, u_hat0, u_hat1, aux, ps, u_out, curl_out, scalars, work, s, nullptr)
+             : step_f32(p, scheme, prm, u_hat, u_hat0, u_hat1, aux, ps, u_out, curl_out, scalars, work, s, nullptr);
 }
 
 int sles_step_noalloc(sles_plan p, int scheme, const sles_step_params* prm, void* u_hat, void* u_hat0,
@@ -115,7 +243,12 @@ int sles_step_drive(sles_plan p, int scheme, const sles_step_params* prm, double
   if (nsteps < 0) return fail("sles_step_drive: nsteps must be >= 0");
   if (!sles_supported_n(p->n)) return fail("sles_step_drive: unsupported grid size");
   hipStream_t s = (hipStream_t)stream;
+  if (nsteps == 0) return 0;
   const DriveArgs drv{state, cfl_first};
+  const long sbytes = 3 * ncplx(p) * cplx_bytes(p);
+  DriveState ds{};
+  if (drive_enter(p, scheme, prm->les_model, u_hat, u_hat0, u_hat1, aux, 1, sbytes, state, 0, work, 0, s, &ds))
+    return -1;
   for (int i = 0; i < nsteps; ++i) {
     // U and curl are outputs only (each step recomputes the physical fields
     // it needs from U_hat, and dt from the z pass's max u): the steps before
@@ -123,12 +256,13 @@ int sles_step_drive(sles_plan p, int scheme, const sles_step_params* prm, double
     const bool last = i + 1 == nsteps;
     void* uo = last ? u_out : nullptr;
     void* co = last ? curl_out : nullptr;
-    const int rc = p->precision == SLES_FP64
-                       ? step_f64(p, scheme, prm, u_hat, u_hat0, u_hat1, aux, uo, co, nullptr, work, s, nullptr, &drv)
-                       : step_f32(p, scheme, prm, u_hat, u_hat0, u_hat1, aux, uo, co, nullptr, work, s, nullptr, &drv);
+    const int rc =
+        p->precision == SLES_FP64
+            ? step_f64(p, scheme, prm, ds.uh, ds.uh0, ds.uh1, ds.aux, ds.ps, uo, co, nullptr, work, s, nullptr, &drv)
+            : step_f32(p, scheme, prm, ds.uh, ds.uh0, ds.uh1, ds.aux, ds.ps, uo, co, nullptr, work, s, nullptr, &drv);
     if (rc) return rc;
   }
-  return 0;
+  return drive_exit(p, scheme, prm->les_model, u_hat, u_hat0, u_hat1, aux, 1, sbytes, state, 0, work, 0, s, ds);
 }
 
 // Members of a batch lie at fixed strides: state fields packed (3 C each),
@@ -158,23 +292,32 @@ int sles_step_drive_batch(sles_plan p, int scheme, const sles_step_params* prm, 
   if (scheme == SLES_SCHEME_ETD2 && !aux) return fail("sles_step_drive_batch: ETD2 needs aux");
   if (nsteps < 0) return fail("sles_step_drive_batch: nsteps must be >= 0");
   if (!sles_supported_n(p->n)) return fail("sles_step_drive_batch: unsupported grid size");
-  const long esz = p->precision == SLES_FP64 ? 16 : 8;
+  if (nsteps == 0) return 0;
+  const long sbytes = 3 * ncplx(p) * cplx_bytes(p);
+  const long wstride = (long)batch_work_stride(p, prm->les_model);
+  const long dstride = (long)(SLES_DRIVE_STATE_SIZE * sizeof(double));
+  hipStream_t s = (hipStream_t)stream;
+  DriveState ds{};
+  if (drive_enter(p, scheme, prm->les_model, u_hat, u_hat0, u_hat1, aux, members, sbytes, state, dstride, work,
+                  wstride, s, &ds))
+    return -1;
   Batch b{};
   b.count = members;
-  b.mb.state = 3 * ncplx(p) * esz;
-  b.mb.work = (long)batch_work_stride(p, prm->les_model);
-  b.mb.drive = (long)(SLES_DRIVE_STATE_SIZE * sizeof(double));
+  b.mb.state = ds.mstride;
+  b.mb.work = wstride;
+  b.mb.drive = dstride;
   b.mb.coeff = prm->les_model ? sgs_coeff : nullptr;
-  hipStream_t s = (hipStream_t)stream;
   const DriveArgs drv{state, cfl_first};
   for (int i = 0; i < nsteps; ++i) {
-    const int rc =
-        p->precision == SLES_FP64
-            ? step_f64(p, scheme, prm, u_hat, u_hat0, u_hat1, aux, nullptr, nullptr, nullptr, work, s, nullptr, &drv, &b)
-            : step_f32(p, scheme, prm, u_hat, u_hat0, u_hat1, aux, nullptr, nullptr, nullptr, work, s, nullptr, &drv, &b);
+    const int rc = p->precision == SLES_FP64
+                       ? step_f64(p, scheme, prm, ds.uh, ds.uh0, ds.uh1, ds.aux, ds.ps, nullptr, nullptr, nullptr,
+                                  work, s, nullptr, &drv, &b)
+                       : step_f32(p, scheme, prm, ds.uh, ds.uh0, ds.uh1, ds.aux, ds.ps, nullptr, nullptr, nullptr,
+                                  work, s, nullptr, &drv, &b);
     if (rc) return rc;
   }
-  return 0;
+  return drive_exit(p, scheme, prm->les_model, u_hat, u_hat0, u_hat1, aux, members, sbytes, state, dstride, work,
+                    wstride, s, ds);
 }
 
 int sles_rk4_stage_states(sles_plan p, const sles_step_params* prm, void* u_hat, void* u_hat0, void* u_hat1,
@@ -185,11 +328,12 @@ int sles_rk4_stage_states(sles_plan p, const sles_step_params* prm, void* u_hat,
   if (prm->dt_device) return fail("sles_rk4_stage_states: needs a host dt (dt_device = 0)");
   if (!sles_supported_n(p->n)) return fail("sles_rk4_stage_states: unsupported grid size");
   hipStream_t s = (hipStream_t)stream;
+  const long ps = ncplx(p) / p->n;  // the caller's arrays: reference layout
   return p->precision == SLES_FP64
-             ? step_f64(p, SLES_SCHEME_RK4, prm, u_hat, u_hat0, u_hat1, nullptr, nullptr, nullptr, nullptr, work, s,
-                        stages)
-             : step_f32(p, SLES_SCHEME_RK4, prm, u_hat, u_hat0, u_hat1, nullptr, nullptr, nullptr, nullptr, work, s,
-                        stages);
+             ? step_f64(p, SLES_SCHEME_RK4, prm, u_hat, u_hat0, u_hat1, nullptr, ps, nullptr, nullptr, nullptr, work,
+                        s, stages)
+             : step_f32(p, SLES_SCHEME_RK4, prm, u_hat, u_hat0, u_hat1, nullptr, ps, nullptr, nullptr, nullptr, work,
+                        s, stages);
 }
 
 int sles_profile_enable(int on) {

@@ -6,7 +6,9 @@ plans for the strided passes, the x-forward/update pass and the z kernel --
 which also fixes each pass's tile: the columns per block follow from the
 plan's threads per line and the 64 KiB LDS tile -- the XCD-aware tile order
 and the fusion of the x-forward with the next x-inverse) on a few RK4 steps
-of a synthetic field, and keeps the fastest.  For N = 256 and 512 it also
+of a synthetic field through the device driver loop, and keeps the fastest;
+then, with those variants, it times the padding of the workspace's kx planes
+(``pad``, in 128-byte lines, N >= 64).  For N = 256 and 512 it also
 times the four tile widths of the generic transforms' strided passes
 (``fft_wide``: twice the columns per tile in the x and/or y pass) on a
 batched forward + backward transform.  Results persist in a JSON table
@@ -33,9 +35,11 @@ import torch
 from . import _lib
 
 TABLE = os.environ.get("SLES_TUNING_TABLE", os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning.json"))
-KNOBS = ("strided_radix4", "xfwd_radix4", "z_radix4", "xcd", "fuse_x", "fft_wide")
+KNOBS = ("strided_radix4", "xfwd_radix4", "z_radix4", "xcd", "fuse_x", "fft_wide", "pad")
 WIDE_SIZES = (256, 512)  # sizes with wide generic-transform tiles
 DEFAULTS = dict(strided_radix4=0, xfwd_radix4=0, z_radix4=0, xcd=1, fuse_x=1)  # a new plan's settings
+PADS = (0, 5, 17, 33)  # workspace plane padding tried, in 128-byte lines
+PAD_MIN_N = 64  # smaller grids keep the compact layout (launch bound)
 RADIX4_SIZES = (12, 24, 32, 48, 64, 96, 128, 192, 256, 384, 512)  # sizes with an alternative line plan
 
 
@@ -94,19 +98,39 @@ def time_fft_wide(domain, wide, reps=5):
 
 
 def time_variant(domain, settings, steps=3, warmup=1):
+    """Mean ms per RK4 step of the device driver loop (sles_step_drive, the
+    path of spectralMethodSolver: CFL dt on the device, U and curl written
+    at the end of the call) with `settings` on the plan; the initial state
+    is restored afterwards."""
+    import ctypes
+    import math
+
     from . import fused
 
-    domain.plan.set_tuning(**settings)
-    for _ in range(warmup):
-        fused.rk4_step(domain, dt=1e-4, physical=True)
+    d = domain
+    d.plan.set_tuning(**settings)
+    u0 = d.U_hat.clone()
+    state = torch.zeros(8, dtype=torch.float64, device=d.device)
+    state[2] = math.nan  # t starts at the first dt
+    state[6] = math.inf  # no output pauses
+    state[7] = math.inf  # no final time
+    prm = fused.params(d, cfl=0.5)
+    w = fused.work(d)  # sized for this variant's layout
+
+    def drive(n):
+        _lib.call("sles_step_drive", d.plan.handle, fused.SCHEME_RK4, ctypes.byref(prm), 0.3, int(n),
+                  d.U_hat.data_ptr(), d.U_hat0.data_ptr(), d.U_hat1.data_ptr(), None, d._U.data_ptr(),
+                  d.curl.data_ptr(), state.data_ptr(), w.data_ptr(), d.stream)
+
+    drive(warmup)
     torch.cuda.synchronize()
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
     e0.record()
-    for _ in range(steps):
-        fused.rk4_step(domain, dt=1e-4, physical=True)
+    drive(steps)
     e1.record()
     e1.synchronize()
+    d.U_hat.copy_(u0)
     return e0.elapsed_time(e1) / steps
 
 
@@ -127,6 +151,17 @@ def tune(N, precision="fp64", les="smagorinsky", steps=3, path=TABLE, verbose=Tr
     results.sort(key=lambda r: r[0])
     best_ms, best = results[0]
     best = dict(best)
+    default_ms = next(ms for ms, st in results if st == DEFAULTS)
+    # the workspace layout, with the best kernel variants (the default
+    # padding of the plan was on while those were timed)
+    if N >= PAD_MIN_N:
+        pads = [(time_variant(d, dict(best, pad=pad), steps=steps), pad) for pad in PADS]
+        if verbose:
+            for ms, pad in pads:
+                print(f"[autotune] N={N} {precision} {les}: pad={pad} -> {ms:.3f} ms/step", flush=True)
+        best_ms, best["pad"] = min(pads)
+    else:
+        best["pad"] = 0
     if N in WIDE_SIZES:
         d.plan.set_tuning(**best)
         wide = [(time_fft_wide(d, w), w) for w in range(4)]
@@ -138,7 +173,7 @@ def tune(N, precision="fp64", les="smagorinsky", steps=3, path=TABLE, verbose=Tr
     table[key(N, precision, d.sgs_model)] = {
         "settings": best,
         "ms_per_step": round(best_ms, 4),
-        "default_ms_per_step": round(next(ms for ms, st in results if st == DEFAULTS), 4),
+        "default_ms_per_step": round(default_ms, 4),
         "device": torch.cuda.get_device_name(),
         "tuned_at": time.strftime("%Y-%m-%d %H:%M:%S"),
     }

@@ -82,8 +82,14 @@ class Group:
         self.state[:, S_T] = math.nan
         self.state[:, S_T_PAUSE] = math.inf
         self.state[:, S_T_FINAL] = math.inf
-        self.work = torch.empty(int(_lib.lib.sles_step_batch_work_bytes(self.plan.handle, d0.sgs_model, M)),
-                                dtype=torch.uint8, device=dev)
+        self.realloc_work()
+
+    def realloc_work(self):
+        """Allocate the batch workspace for the plan's current layout (its
+        size follows the plan's SLES_TUNE_PAD: call again after changing it)."""
+        d0 = self.members[0]
+        nbytes = int(_lib.lib.sles_step_batch_work_bytes(self.plan.handle, d0.sgs_model, len(self.members)))
+        self.work = torch.empty(nbytes, dtype=torch.uint8, device=d0.device)
 
     @staticmethod
     def _scheme(d):
