@@ -183,7 +183,7 @@ def run_gpu(N, precision, steps, warmup, profile_steps=3, chunk=10):
         _lib.call("sles_profile_enable", 0)
         kern = {KINDS[i]: (ms_k[i], cnt_k[i]) for i in range(7)}
     out = dict(ms_total=ms, ms_total_step_each=ms_each, steps=steps, dt_last=float(st[1]),
-               umax_last=float(st[0]), kernels=kern, ok=ok)
+               umax_last=float(st[0]), kernels=kern, ok=ok, tuning=d.plan.get_tuning())
     del d
     torch.cuda.empty_cache()
     return out
@@ -338,19 +338,20 @@ def tgv_cpu(N=64, steps=20):
             "sample": f"{steps} RK4 steps of the 64^3 Taylor-Green case, numpy pocketfft, {t:.2f} s"}
 
 
-def x_forward_pattern_ceiling(N, precision):
+def x_forward_pattern_ceiling(N, precision, pad):
     """The x-forward + update pass's own access pattern with no arithmetic
-    (sles_stream_probe, the radix-4 x-forward tile, XCD order): the mean over
-    the 4 RK4 stages' traffic mixes (9+9, 15+6, 15+6, 12+6 fields read +
-    written) of the algorithmic GB/s -- the bandwidth this tile shape and
-    traffic reach on this device, beside the 8 TB/s peak."""
+    (sles_stream_probe, the radix-4 x-forward tile, XCD order, the step's
+    padded plane layout): the mean over the 4 RK4 stages' traffic mixes
+    (9+9, 15+6, 15+6, 12+6 fields read + written) of the algorithmic GB/s --
+    the bandwidth this tile shape and traffic reach on this device, beside
+    the 8 TB/s peak."""
     from sles.studies import stream_ceiling
 
-    r = stream_ceiling(N, precision, mixes=((9, 9), (15, 6), (12, 6)), tiles=(0,), verbose=False)
+    r = stream_ceiling(N, precision, mixes=((9, 9), (15, 6), (12, 6)), tiles=(0,), verbose=False, pad=pad)
     R, C = field_bytes(N, precision)
     ms = (r["tile0_9r9w"]["ms"] + 2 * r["tile0_15r6w"]["ms"] + r["tile0_12r6w"]["ms"]) / 4
     nbytes = xfwd_mean_fields(True) * C
-    return {"GBps": round(nbytes / (ms * 1e-3) / 1e9, 1), "avg_launch_ms": round(ms, 4),
+    return {"GBps": round(nbytes / (ms * 1e-3) / 1e9, 1), "avg_launch_ms": round(ms, 4), "pad_lines": pad,
             "kernel": "sles_stream_probe tile 0 (loads and stores of the x-forward tile, no arithmetic)"}
 
 
@@ -504,7 +505,7 @@ def main(argv=None):
     ms_step = r64["ms_total"] / a.steps
     value = 1e3 / ms_step
     roof, per_kernel = roofline_of(r64, N, "fp64")
-    roof["pattern_ceiling"] = x_forward_pattern_ceiling(N, "fp64")
+    roof["pattern_ceiling"] = x_forward_pattern_ceiling(N, "fp64", r64["tuning"]["pad"])
     step_bytes = step_bytes_minimal(N, "fp64", True)
     log(f"[bench] fp64 {value:.2f} steps/s ({ms_step:.3f} ms/step); dominant {roof['kernel']} "
         f"{roof['achieved']} GB/s")
@@ -515,6 +516,7 @@ def main(argv=None):
         "step_roofline_frac": round(step_bytes * value / (HBM_PEAK_GBS * 1e9), 4),
         "step_algorithmic_GB": round(step_bytes / 1e9, 3),
         "kernels_fp64": per_kernel,
+        "tuning_fp64": r64["tuning"],
         "dt_last": r64["dt_last"],
         "fp64_valid": r64["ok"],
     }

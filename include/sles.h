@@ -45,7 +45,7 @@ int sles_plan_info(sles_plan plan, int* n, int* nh, int* precision);
  * sles/autotune.py, which persists its choices in a JSON table) and of the
  * generic transforms.  Defaults: 0 for the three plan choices, 1 for
  * SLES_TUNE_XCD and SLES_TUNE_FUSE_X; SLES_TUNE_FFT_WIDE 3 at N = 512,
- * 1 (FP64) / 2 (FP32) at N = 256, 0 otherwise; SLES_TUNE_PAD 17 for
+ * 1 (FP64) / 2 (FP32) at N = 256, 0 otherwise; SLES_TUNE_PAD 5 for
  * N >= 128, 0 otherwise.  No counterpart in the reference.
  *   SLES_TUNE_STRIDED_RADIX4  x/y inverse + y forward passes use the radix-4
  *                             line plan (4 points per thread) instead of radix-8
@@ -413,8 +413,10 @@ int sles_lincomb(sles_plan plan, int nfields, double a, const void* x, double b,
 /* ---- measurement (no counterpart in the reference) ----
  * The loads and stores of a strided x pass with no arithmetic: every thread
  * of a tile of consecutive columns reads its rows of the nin spectral
- * fields at `in` (contiguous) and writes their sum to the nout fields at
- * `out`.  Its bandwidth is the ceiling of a pass with that tile shape:
+ * fields at `in` and writes their sum to the nout fields at `out`; fields
+ * in the fused step's workspace layout (kx planes N (N/2+1) values +
+ * SLES_TUNE_PAD 128-byte lines apart, fields N planes apart).  Its
+ * bandwidth is the ceiling of a pass with that tile shape:
  * tile 0 the radix-4 x-forward tile (FP64 8 columns x 64 threads, 128-byte
  * row runs; FP32 16 x 64), 1 16-column tiles (the radix-8 strided shape),
  * 2 32-column tiles.

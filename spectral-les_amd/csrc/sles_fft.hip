@@ -82,7 +82,7 @@ __global__ void __launch_bounds__(ZShape<N>::THREADS)
 
 template <typename T, int N>
 __global__ void __launch_bounds__(ZShape<N>::THREADS)
-    k_z_c2r(const cpx<T>* __restrict__ in, long plane, T* __restrict__ out, long nrows, T scale,
+    k_z_c2r(const cpx<T>* __restrict__ in, long plane, int split, T* __restrict__ out, long nrows, T scale,
             const cpx<T>* __restrict__ tw) {
   using S = ZShape<N>;
   constexpr int P = S::P, NT = S::NT, NH = S::NH;
@@ -98,12 +98,19 @@ __global__ void __launch_bounds__(ZShape<N>::THREADS)
   // m < P/2, plus k = N/2 in thread 0); the Hermitian completion k > N/2 takes
   // conj of bin N - k from the line's LDS row (unpermuted layout).
   cpx<T> ha[P / 2 + 1], hb[P / 2 + 1];
+  // bin k of row r: plane r / N; row r % N at (r % N) (N/2+1) + k, or split:
+  // at (r % N) N/2 + k for k < N/2 and N N/2 + r % N for k = N/2
+  auto bin = [&](long r, int k) {
+    const long x = r / N, y = r % N;
+    if (!split) return x * plane + y * NH + k;
+    return x * plane + (k < N / 2 ? y * (N / 2) + k : (long)N * (N / 2) + y);
+  };
 #pragma unroll
   for (int m = 0; m <= P / 2; ++m) {
     const int k = t + m * NT;
     if (k <= N / 2) {
-      ha[m] = in[(rla / N) * plane + (rla % N) * NH + k];
-      hb[m] = in[(rlb / N) * plane + (rlb % N) * NH + k];
+      ha[m] = in[bin(rla, k)];
+      hb[m] = in[bin(rlb, k)];
       if (!va) ha[m] = {T(0), T(0)};
       if (!vb) hb[m] = {T(0), T(0)};
     }
@@ -236,13 +243,13 @@ static int launch_z_r2c(const sles_plan_s* p, const void* in, void* out, long nr
 }
 
 template <typename T, int N>
-static int launch_z_c2r(const sles_plan_s* p, const void* in, long plane, void* out, long nrows, double scale,
-                        hipStream_t s) {
+static int launch_z_c2r(const sles_plan_s* p, const void* in, long plane, int split, void* out, long nrows,
+                        double scale, hipStream_t s) {
   using S = ZShape<N>;
   const long pairs = (nrows + 1) / 2;
   const long blocks = (pairs + S::LPB - 1) / S::LPB;
   hipLaunchKernelGGL((k_z_c2r<T, N>), dim3((unsigned)blocks), dim3(S::THREADS), 0, s,
-                     (const cpx<T>*)in, plane, (T*)out, nrows, (T)scale, (const cpx<T>*)p->tw);
+                     (const cpx<T>*)in, plane, split, (T*)out, nrows, (T)scale, (const cpx<T>*)p->tw);
   return check_launch("z_c2r");
 }
 
@@ -313,7 +320,7 @@ static int fft_c2r_t(const sles_plan_s* p, const void* in, void* out, void* work
   case NN:                                                                                 \
     rc = launch_strided<T, NN>(p, in, work, 1, (long)n * p->nh, C, batch, +1, 1.0, wx, s);  \
     if (!rc) rc = launch_strided<T, NN>(p, work, work, (long)n, (long)p->nh, C, batch, +1, 1.0, wy, s); \
-    if (!rc) rc = launch_z_c2r<T, NN>(p, work, (long)n * p->nh, out, rows, scale, s);      \
+    if (!rc) rc = launch_z_c2r<T, NN>(p, work, (long)n * p->nh, 0, out, rows, scale, s);   \
     break;
   switch (n) {
     SLES_FOR_EACH_N(SLES_CASE)
@@ -325,12 +332,13 @@ static int fft_c2r_t(const sles_plan_s* p, const void* in, void* out, void* work
 }
 
 // Internal entry points shared with the fused time-step code.
-int z_c2r_rows(const sles_plan_s* p, const void* in, long plane, void* out, long nrows, double scale, hipStream_t s) {
+int z_c2r_rows(const sles_plan_s* p, const void* in, long plane, int split, void* out, long nrows, double scale,
+               hipStream_t s) {
   int rc = 0;
 #define SLES_CASE(NN)                                                                         \
   case NN:                                                                                    \
-    rc = p->precision == SLES_FP64 ? launch_z_c2r<double, NN>(p, in, plane, out, nrows, scale, s) \
-                                   : launch_z_c2r<float, NN>(p, in, plane, out, nrows, scale, s);  \
+    rc = p->precision == SLES_FP64 ? launch_z_c2r<double, NN>(p, in, plane, split, out, nrows, scale, s) \
+                                   : launch_z_c2r<float, NN>(p, in, plane, split, out, nrows, scale, s);  \
     break;
   switch (p->n) {
     SLES_FOR_EACH_N(SLES_CASE)
@@ -368,11 +376,11 @@ int sles_plan_create(sles_plan* out, int n, int precision) {
   // batch of 3 fields): both passes at 512, the x pass (FP64) or the y pass
   // (FP32) at 256
   p->tune_fft_wide = n == 512 ? 3 : n == 256 ? (precision == SLES_FP64 ? 1 : 2) : 0;
-  // fused-step workspace planes padded by 17 lines (2176 B) from N = 128 on,
+  // fused-step workspace planes padded by 5 lines (640 B) from N = 128 on,
   // where the strided x passes stream faster with it (256^3 FP64 access
-  // pattern of the x-forward pass 3.8 -> 4.7-4.9 TB/s); the small grids are
+  // pattern of the x-forward pass 3.8 -> 4.9 TB/s); the small grids are
   // launch bound and keep the compact layout
-  p->tune_pad = n >= 128 ? 17 : 0;
+  p->tune_pad = n >= 128 ? 5 : 0;
   (void)hipGetDevice(&p->device);
   step_scratch_init(p->device);
   // Twiddles exp(-2 pi i e/n) evaluated in long double and rounded once.

@@ -118,8 +118,11 @@ using WideShape = StridedShape<T, N, PL, (sizeof(T) == 4 && PL::NT >= 64) ? 1024
 void step_scratch_init(int device);
 
 // c2r along z of nrows half-spectrum rows (row r: plane r / N, row r % N of
-// planes `plane` elements apart) into nrows real rows of N.
-int z_c2r_rows(const sles_plan_s* p, const void* in, long plane, void* out, long nrows, double scale, hipStream_t s);
+// planes `plane` elements apart, arranged as the reference's (split = 0) or
+// as the fused step's workspace (split = 1: rows of N/2, then the Nyquist
+// column)) into nrows real rows of N.
+int z_c2r_rows(const sles_plan_s* p, const void* in, long plane, int split, void* out, long nrows, double scale,
+               hipStream_t s);
 
 }  // namespace sles
 

@@ -107,14 +107,76 @@ template <int N, int TK>
 struct Cols {
   static constexpr int NH = N / 2 + 1;
   static constexpr long M = (long)N * NH;
-  int c;
+  int c;    // column of the tile
+  int t;    // the thread's slot in that column's line (points t + m NT)
+  long g0;  // the tile's first column (uniform over the block)
   long g;
   bool valid;
-  __device__ __forceinline__ explicit Cols(int xcd) {
-    c = threadIdx.x % TK;
-    g = tile_id(xcd) * TK + c;
+  // Lanes run across the tile's columns (c = threadIdx.x mod TK), so that a
+  // row of the tile is one contiguous run -- or, with `along` (uniform over
+  // the block), along each column's line (t = threadIdx.x mod nt), for
+  // tiles whose lines are contiguous in memory.
+  __device__ __forceinline__ explicit Cols(int xcd, int nt = 1, bool along = false) {
+    if (along) {
+      c = threadIdx.x / nt;
+      t = threadIdx.x % nt;
+    } else {
+      c = threadIdx.x % TK;
+      t = threadIdx.x / TK;
+    }
+    g0 = tile_id(xcd) * TK;
+    g = g0 + c;
     valid = g < M;
     if (!valid) g = 0;
+  }
+};
+
+
+// Arrangement of one kx plane of the workspace (and of the drivers' copies
+// of the RK state): the rows kz < N/2 of every ky first (ky * N/2 + kz), then
+// the N values of the Nyquist column kz = N/2 (N * N/2 + ky).  Every row the
+// y passes and the z pass touch then starts on a multiple of N/2 values
+// (aligned 128-byte runs) instead of every N/2 + 1, and the x passes' tiles of
+// consecutive plane columns are still contiguous runs.  The reference layout
+// (ky * (N/2+1) + kz) stays the caller's.
+template <int N>
+struct Split {
+  static constexpr int NR = N / 2, NH = N / 2 + 1;
+  static constexpr long MAIN = (long)N * NR;  // the rows; the Nyquist column follows
+  // (ky index, kz) of plane column j
+  __device__ __forceinline__ static void at(long j, int& iy, int& kz) {
+    if (j < MAIN) {
+      iy = (int)(j / NR);
+      kz = (int)(j % NR);
+    } else {
+      iy = (int)(j - MAIN);
+      kz = NR;
+    }
+  }
+  // plane column of (ky index, kz)
+  __device__ __forceinline__ static long col(int iy, int kz) { return kz < NR ? (long)iy * NR + kz : MAIN + iy; }
+};
+
+// Column g of an x-pass tile.  With the RK state in the workspace
+// arrangement (a driver's copies, ssplit != 0) the tiles run over the split
+// plane's columns, so the workspace and the state accesses are both
+// contiguous runs; with the state in the reference layout (the caller's
+// arrays) they run over the reference plane's columns, which keeps those
+// runs contiguous, and the workspace column follows.
+template <int N>
+struct XCol {
+  long jw, js;  // plane column in the workspace, in the state arrays
+  int iy, kz;   // ky index and kz of the column
+  __device__ __forceinline__ XCol(long g, int ssplit) {
+    if (ssplit) {
+      Split<N>::at(g, iy, kz);
+      jw = js = g;
+    } else {
+      iy = (int)(g / (N / 2 + 1));
+      kz = (int)(g % (N / 2 + 1));
+      js = g;
+      jw = Split<N>::col(iy, kz);
+    }
   }
 };
 
@@ -128,13 +190,46 @@ struct XRows {
   __device__ __forceinline__ eidx operator()(int m) const { return base + (eidx)m * step; }
 };
 
+// A y-pass tile that lies in the Nyquist part of the workspace arrangement
+// (whole tiles only: TK divides N N/2).  Its lines -- the Nyquist column of
+// TK planes -- are contiguous, so its lanes run along them.
+template <int N, int TK>
+__device__ __forceinline__ bool y_nyquist_tile(int xcd) {
+  return (Split<N>::MAIN % TK == 0) && tile_id(xcd) * TK >= Split<N>::MAIN;
+}
+
+// Rows of column q of a y-pass tile (q < N (N/2+1)) in the workspace
+// arrangement: plane q / (N/2) at kz = q mod N/2 (q < N N/2, row stride N/2),
+// or the Nyquist column of plane q - N N/2, whose ky values are contiguous
+// (row stride 1).  Thread t of the column holds rows t + m NT.  When TK
+// divides N N/2 a tile lies wholly in one of the two parts and the stride --
+// taken from the tile's first column g0 -- is uniform over the block (scalar
+// per-slot offsets); otherwise (small grids) it is per column.  body(at)
+// runs the pass with at(m) = element offset of slot m in a field.  (One copy
+// of the body: two copies, each with a compile-time stride, measured 1-3 %
+// slower.)
+template <int N, int TK, int NT, class Body>
+__device__ __forceinline__ void y_rows(long q, long g0, long pw, int t, Body&& body) {
+  constexpr int NR = N / 2;
+  constexpr long MAIN = Split<N>::MAIN;
+  const bool nyq = (MAIN % TK == 0) ? g0 >= MAIN : q >= MAIN;
+  const long qq = (nyq && q < MAIN) ? MAIN : q;  // lanes past the last column stay in bounds
+  const eidx b = nyq ? (eidx)(qq - MAIN) * pw + MAIN + t : (eidx)(q / NR) * pw + q % NR + (eidx)t * NR;
+  const eidx st = nyq ? (eidx)NT : (eidx)NT * NR;
+  body([=](int m) { return b + (eidx)m * st; });
+}
+
+// kz of column q of a y pass (N/2 in the Nyquist part)
+template <int N>
+__device__ __forceinline__ int y_kz(long q) { return q < Split<N>::MAIN ? (int)(q % (N / 2)) : N / 2; }
+
 // ---------------------------------------------------------------------------
 // 1. x-inverse
 // ---------------------------------------------------------------------------
 template <typename T, int N, int LES, class PL>
 __global__ void __launch_bounds__((WideShape<T, N, PL>::THREADS))
-    k_xinv(const cpx<T>* __restrict__ uh, long ps, cpx<T>* __restrict__ W, long pw, const cpx<T>* __restrict__ tw,
-           int xcd, Members mb) {
+    k_xinv(const cpx<T>* __restrict__ uh, long ps, int ssplit, cpx<T>* __restrict__ W, long pw,
+           const cpx<T>* __restrict__ tw, int xcd, Members mb) {
   uh = member_ptr(uh, mb.state);
   W = member_ptr(W, mb.work);
   using S = WideShape<T, N, PL>;
@@ -146,7 +241,8 @@ __global__ void __launch_bounds__((WideShape<T, N, PL>::THREADS))
   const int t = threadIdx.x / TK;
   const TwRegs<T, PL> twr(tw, t);  // read once per block, used by every transform
   const LdsColumn<TK> map{col.c};
-  const XRows rs(t, NT, ps, col.g), rw(t, NT, pw, col.g);  // U_hat rows, W rows
+  const XCol<N> xc(col.g, ssplit);
+  const XRows rs(t, NT, ps, xc.js), rw(t, NT, pw, xc.jw);  // U_hat rows, W rows
   const long CS = N * ps, C = N * pw;
   cpx<T> v[P];
   if constexpr (LES) {
@@ -210,14 +306,13 @@ __global__ void __launch_bounds__((WideShape<T, N, PL>::THREADS))
   constexpr int PPB = N * TK;  // ping-pong exchange buffers
   using FFT = LineFFTP<T, PL, +1, false, PPB>;
   __shared__ cpx<T> lds[N * TK + PPB];
-  const Cols<N, TK> col(xcd);
-  const int t = threadIdx.x / TK;
+  const Cols<N, TK> col(xcd, NT, y_nyquist_tile<N, TK>(xcd));
+  const int t = col.t;
   const TwRegs<T, PL> twr(tw, t);  // read once per block, used by every transform
   const LdsColumn<TK> map{col.c};
-  const long ix = col.g / NH, iz = col.g % NH;
-  const eidx row0 = (eidx)ix * pw + (eidx)t * NH + (eidx)iz;
-  auto at = [&](int m) { return row0 + (eidx)(m * NT * NH); };
+  const long iz = y_kz<N>(col.g);
   const long C = N * pw;
+  y_rows<N, TK, NT>(col.g, col.g0, pw, t, [&](auto at) {
   cpx<T> v[P];
   if constexpr (LES) {
 #pragma unroll 1
@@ -283,6 +378,7 @@ __global__ void __launch_bounds__((WideShape<T, N, PL>::THREADS))
     fft_store(u, 1);
     fft_store(w, 2);
   }
+  });
 }
 
 // ---------------------------------------------------------------------------
@@ -304,38 +400,109 @@ template <typename T, int N, class PL>
 constexpr int ZPPB = WaveLocal<PL>::value ? 0 : ZLines<PL>::LPB * LdsLine<T, N>::STRIDE;
 
 // Line pairs Z_k = A_k + i B_k of two half-spectra A, B (uniform field bases,
-// line starting at element row0): pair_issue loads bins k <= N/2 once,
+// line at row0, Nyquist bins staged in LDS): pair_issue loads bins k < N/2 once,
 // pair_finish completes the Hermitian half through LDS, multiplies by i*kz
 // where asked and drops the imaginary parts of the kz = 0 and N/2 bins (as
 // c2r does).
 // Split in two so that the next pair's loads are issued before the current
 // pair's transform.
-template <typename T, int P>
+// NQ: the line's Nyquist bin is staged in LDS (see z_row0) instead of held
+// in registers by thread 0.
+template <typename T, int P, bool NQ>
 struct PairRaw {
-  cpx<T> a[P / 2 + 1], b[P / 2 + 1];  // bins k = t + m*NT <= N/2 (m = P/2: thread 0 only)
+  static constexpr int S = NQ ? P / 2 : P / 2 + 1;
+  cpx<T> a[S], b[S];  // bins k = t + m*NT < N/2 (and k = N/2, thread 0, without NQ)
 };
-template <typename T, int N, int P, int NT>
-__device__ __forceinline__ void pair_issue(PairRaw<T, P>& r, const cpx<T>* A, const cpx<T>* B, unsigned row0, int t) {
+
+// A z line (plane x, row y) of the workspace arrangement: bins k < N/2 at
+// x pw + y N/2 + k (row0 + k, 32-bit element offsets), the Nyquist bin
+// k = N/2 in the plane's Nyquist column at x pw + N N/2 + y.  Only thread 0
+// of a line holds that bin.  In FP32 the z kernels stage the Nyquist bins of
+// their lines through LDS (loaded together at the start, stored together at
+// the end), so that no second address and no extra slot per line stay in
+// registers: the radix-8 kernels kept their 2 waves per SIMD that way (the
+// direct form needed more than 256 VGPRs); in FP64 thread 0 loads and stores
+// the bin directly (measured 6 % faster there: 256^3 z pass 0.502 against
+// 0.534 ms, 512^3 4.67 against 4.95 ms).
+template <int N>
+__device__ __forceinline__ unsigned z_row0(long r, long pw) {
+  return (unsigned)((r / N) * pw + (r % N) * (N / 2));
+}
+template <int N>
+__device__ __forceinline__ unsigned z_nyquist(long r, long pw) {
+  return (unsigned)((r / N) * pw + Split<N>::MAIN + r % N);
+}
+
+// Nyquist bins of fields f < NF (field stride C) for the block's LPB lines,
+// destined for nq[f * LPB + line] (lines past the last row read row 0).
+// When the block has at least NF LPB lanes, lane i issues the load of value
+// i (every lane loads, branch-free; lanes past NF LPB read value 0 again) and
+// nyquist_stage stores it later, so the load's latency overlaps whatever is
+// issued in between; otherwise nyquist_issue loads them all into nq itself.
+template <typename T, int N, int LPB, int NF, int THREADS>
+__device__ __forceinline__ cpx<T> nyquist_issue(cpx<T>* nq, const cpx<T>* W, long C, long pw) {
+  auto value = [&](int i) {
+    const int f = i / LPB, l = i % LPB;
+    long r = (long)blockIdx.x * LPB + l;
+    if (r >= (long)N * N) r = 0;
+    return ld32(W + f * C, z_nyquist<N>(r, pw));
+  };
+  if constexpr (NF * LPB <= THREADS) {
+    return value(threadIdx.x < NF * LPB ? (int)threadIdx.x : 0);
+  } else {
+    for (int i = threadIdx.x; i < NF * LPB; i += THREADS) nq[i] = value(i);
+    return cpx<T>{T(0), T(0)};
+  }
+}
+// nq must have room for max(NF LPB, THREADS) values; ends with the line
+// synchronisation
+template <typename T, int LPB, int NF, int THREADS, bool WAVE>
+__device__ __forceinline__ void nyquist_stage(cpx<T>* nq, cpx<T> v) {
+  if constexpr (NF * LPB <= THREADS) nq[threadIdx.x] = v;  // lanes past NF LPB write unused slots
+  line_sync<WAVE>();
+}
+// nq[f * LPB + line] -> the Nyquist bins of fields f < nf (valid lines)
+template <typename T, int N, int LPB, bool WAVE>
+__device__ __forceinline__ void nyquist_store(const cpx<T>* nq, cpx<T>* W, long C, int nf, long pw) {
+  line_sync<WAVE>();
+  for (int i = threadIdx.x; i < nf * LPB; i += blockDim.x) {
+    const int f = i / LPB, l = i % LPB;
+    const long r = (long)blockIdx.x * LPB + l;
+    if (r < (long)N * N) st32(W + f * C, z_nyquist<N>(r, pw), nq[i]);
+  }
+}
+
+template <typename T, int N, int P, int NT, bool NQ>
+__device__ __forceinline__ void pair_issue(PairRaw<T, P, NQ>& r, const cpx<T>* A, const cpx<T>* B, unsigned row0,
+                                           unsigned nyq, int t) {
 #pragma unroll
-  for (int m = 0; m <= P / 2; ++m) {
-    const int k = t + m * NT;
-    if (k <= N / 2) {
-      r.a[m] = ld32(A, row0 + k);
-      r.b[m] = ld32(B, row0 + k);
+  for (int m = 0; m < P / 2; ++m) {
+    r.a[m] = ld32(A, row0 + t + m * NT);
+    r.b[m] = ld32(B, row0 + t + m * NT);
+  }
+  if constexpr (!NQ) {
+    if (t == 0) {
+      r.a[P / 2] = ld32(A, nyq);
+      r.b[P / 2] = ld32(B, nyq);
     }
   }
 }
 // Each half-spectrum element is loaded once; the completion k > N/2 takes
-// conj of bin N - k from the line's LDS row (pmap: the unpermuted layout).
-// The row must be free on entry (the previous transform ended with its
-// barrier); it is free again on return.
-template <typename T, int N, int P, int NT, bool WAVE, class Map>
-__device__ __forceinline__ void pair_finish(cpx<T> (&v)[P], const PairRaw<T, P>& r, bool dzA, bool dzB, int t,
-                                            cpx<T>* lds, const Map& pmap) {
-  auto complete = [&](const cpx<T>(&h)[P / 2 + 1], bool dz, bool second) {
+// conj of bin N - k from the line's LDS row (pmap: the unpermuted layout);
+// the Nyquist bins nqa, nqb come from the LDS staging area.  The row must
+// be free on entry (the previous transform ended with its barrier); it is
+// free again on return.
+template <typename T, int N, int P, int NT, bool WAVE, bool NQ, class Map>
+__device__ __forceinline__ void pair_finish(cpx<T> (&v)[P], const PairRaw<T, P, NQ>& r, const cpx<T>& nqa,
+                                            const cpx<T>& nqb, bool dzA, bool dzB, int t, cpx<T>* lds,
+                                            const Map& pmap) {
+  auto complete = [&](const cpx<T>(&h)[PairRaw<T, P, NQ>::S], const cpx<T>& nq, bool dz, bool second) {
 #pragma unroll
-    for (int m = 0; m <= P / 2; ++m)
-      if (t + m * NT <= N / 2) lds[pmap(t + m * NT)] = h[m];
+    for (int m = 0; m < P / 2; ++m) lds[pmap(t + m * NT)] = h[m];
+    if constexpr (NQ)
+      lds[pmap(N / 2)] = nq;  // by every lane of the line (the same value): no divergent branch
+    else if (t == 0)
+      lds[pmap(N / 2)] = h[P / 2];
     line_sync<WAVE>();
 #pragma unroll
     for (int m = 0; m < P; ++m) {
@@ -351,8 +518,8 @@ __device__ __forceinline__ void pair_finish(cpx<T> (&v)[P], const PairRaw<T, P>&
     }
     line_sync<WAVE>();
   };
-  complete(r.a, dzA, false);
-  complete(r.b, dzB, true);
+  complete(r.a, nqa, dzA, false);
+  complete(r.b, nqb, dzB, true);
 }
 
 // After a forward FFT of a + i b held in v (natural layout), split into the
@@ -411,38 +578,54 @@ __global__ void __launch_bounds__(ZLines<PL>::THREADS)
   using IFFT = LineFFTP<T, PL, +1, WV, ZPPB<T, N, PL>>;
   using FFT = LineFFTP<T, PL, -1, WV, ZPPB<T, N, PL>>;
   __shared__ cpx<T> lds[LPB * LdsLine<T, N>::STRIDE + ZPPB<T, N, PL>];
+  constexpr int TH = ZLines<PL>::THREADS;
+  constexpr bool NQ = sizeof(T) == 4;  // Nyquist bins staged in LDS (z_row0)
+  __shared__ cpx<T> nq[!NQ ? 1 : 9 * LPB > TH ? 9 * LPB : TH];  // Nyquist bins of the block's lines: 9 inputs, then 8 outputs
   const int line = threadIdx.x / NT, t = threadIdx.x % NT;
   const long row = (long)blockIdx.x * LPB + line;
   const bool valid = row < (long)N * N;
-  const long rv = valid ? row : 0;  // (x, y) line: plane x = rv / N, row y = rv % N
-  const unsigned row0 = (unsigned)((rv / N) * pw + (rv % N) * NH);
+  const unsigned row0 = z_row0<N>(valid ? row : 0, pw);  // (x, y) line: plane x = row / N, row y = row % N
   const long C = N * pw;
   const LdsLine<T, N> map{line * LdsLine<T, N>::STRIDE};
   const TwRegs<T, PL> twr(tw, t);  // read once, used by all 11 transforms
   cpx<T>* w[9];
 #pragma unroll
   for (int i = 0; i < 9; ++i) w[i] = W + i * C;
+  const unsigned nyq_addr = z_nyquist<N>(valid ? row : 0, pw);  // thread 0's Nyquist bin, without NQ
+  auto nyq = [&](int f) -> cpx<T>& { return nq[NQ ? f * LPB + line : 0]; };
+  // Nyquist bin of output field f (thread 0 of each line holds it)
+  auto put_nyq = [&](int f, const cpx<T>& val) {
+    if constexpr (NQ) {
+      if (t == 0) nyq(f) = val;
+    } else {
+      if (valid && t == 0) st32(w[f], nyq_addr, val);
+    }
+  };
 
   // ---- inverse: 6 complex transforms -> 12 physical fields (scaled 1/N^3)
   T u0[P], u1[P], u2[P], a11[P], a12[P], a13[P], a21[P], a22[P], a23[P], a31[P], a32[P], a33[P];
   cpx<T> v[P];
-  // pair i+1 is loaded while pair i is transformed
-  PairRaw<T, P> pr;
-  pair_issue<T, N, P, NT>(pr, w[0], w[1], row0, t);
-#define SLES_INV(DA, DB, NA, NB, RA, RB)                           \
-  pair_finish<T, N, P, NT, WV>(v, pr, DA, DB, t, lds, map.plain()); \
-  if (NA) pair_issue<T, N, P, NT>(pr, NA, NB, row0, t);            \
-  IFFT::run(v, lds, map, t, twr);                                  \
-  _Pragma("unroll") for (int m = 0; m < P; ++m) {                  \
-    RA[m] = scale * v[m].x;                                        \
-    RB[m] = scale * v[m].y;                                        \
+  // pair i+1 is loaded while pair i is transformed; the Nyquist bins' loads
+  // follow the first pair's, so their latencies overlap
+  PairRaw<T, P, NQ> pr;
+  cpx<T> nqv{};
+  if constexpr (NQ) nqv = nyquist_issue<T, N, LPB, 9, TH>(nq, W, C, pw);
+  pair_issue<T, N, P, NT>(pr, w[0], w[1], row0, nyq_addr, t);
+  if constexpr (NQ) nyquist_stage<T, LPB, 9, TH, WV>(nq, nqv);
+#define SLES_INV(FA, FB, DA, DB, NA, NB, RA, RB)                                     \
+  pair_finish<T, N, P, NT, WV>(v, pr, nyq(FA), nyq(FB), DA, DB, t, lds, map.plain()); \
+  if (NA >= 0) pair_issue<T, N, P, NT>(pr, w[NA], w[NB], row0, nyq_addr, t);           \
+  IFFT::run(v, lds, map, t, twr);                                                    \
+  _Pragma("unroll") for (int m = 0; m < P; ++m) {                                    \
+    RA[m] = scale * v[m].x;                                                          \
+    RB[m] = scale * v[m].y;                                                          \
   }
-  SLES_INV(false, false, w[2], w[0], u0, u1)
-  SLES_INV(false, true, w[1], w[2], u2, a31)
-  SLES_INV(true, true, w[3], w[4], a32, a33)
-  SLES_INV(false, false, w[5], w[6], a11, a12)
-  SLES_INV(false, false, w[7], w[8], a13, a21)
-  SLES_INV(false, false, (cpx<T>*)nullptr, (cpx<T>*)nullptr, a22, a23)
+  SLES_INV(0, 1, false, false, 2, 0, u0, u1)
+  SLES_INV(2, 0, false, true, 1, 2, u2, a31)
+  SLES_INV(1, 2, true, true, 3, 4, a32, a33)
+  SLES_INV(3, 4, false, false, 5, 6, a11, a12)
+  SLES_INV(5, 6, false, false, 7, 8, a13, a21)
+  SLES_INV(7, 8, false, false, -1, -1, a22, a23)
 #undef SLES_INV
 
   // ---- pointwise: omega, u x omega, nu_R, tau = nu_R S_ij, max(u)
@@ -482,54 +665,68 @@ __global__ void __launch_bounds__(ZLines<PL>::THREADS)
   }
   if (umax_part) block_max_store(valid ? (double)umax : -INFINITY, umax_part);
 
-  // ---- forward: 5 complex transforms -> 8 output half-spectra
-  cpx<T> A[KH], B[KH], t13[KH], t23[KH];
-  auto kok = [&](int m) { return t + m * NT <= N / 2; };
+  // ---- forward: 5 complex transforms -> 8 output half-spectra; slots
+  // m < P/2 (bins below N/2) go to memory, thread 0's Nyquist bin (slot P/2)
+  // to the LDS staging area (outputs overwrite the inputs' slots, whose
+  // values are no longer needed), stored with the others at the end
+  cpx<T> A[KH], B[KH], t13[P / 2], t23[P / 2];
+  constexpr T KN2 = T(N);  // 2 kz at the Nyquist bin
 #define SLES_FWD(QA, QB)                                     \
   _Pragma("unroll") for (int m = 0; m < P; ++m) v[m] = {QA[m], QB[m]}; \
   FFT::run(v, lds, map, t, twr);                             \
   split_pair<T, N, P, NT, WV>(v, lds, map.plain(), t, A, B);
   SLES_FWD(q4, q5)  // tau_xy, tau_xz
 #pragma unroll
-  for (int m = 0; m < KH; ++m) {
+  for (int m = 0; m < P / 2; ++m) {
     t13[m] = B[m];
-    if (valid && kok(m)) {
+    if (valid) {
       st32(w[4], row0 + t + m * NT, A[m]);
       st32(w[5], row0 + t + m * NT, B[m]);
     }
   }
+  put_nyq(4, A[P / 2]);
+  put_nyq(5, B[P / 2]);
+  const cpx<T> n13 = B[P / 2];  // without NQ, thread 0 keeps it for N_x
   SLES_FWD(q6, q7)  // tau_yy, tau_yz
 #pragma unroll
-  for (int m = 0; m < KH; ++m) {
+  for (int m = 0; m < P / 2; ++m) {
     t23[m] = B[m];
-    if (valid && kok(m)) {
+    if (valid) {
       st32(w[6], row0 + t + m * NT, A[m]);
       st32(w[7], row0 + t + m * NT, B[m]);
     }
   }
+  put_nyq(6, A[P / 2]);
+  put_nyq(7, B[P / 2]);
+  const cpx<T> n23 = B[P / 2];
   SLES_FWD(q2, q8)  // N_z, tau_zz
 #pragma unroll
-  for (int m = 0; m < KH; ++m) {
+  for (int m = 0; m < P / 2; ++m) {
     const T kz2 = T(2) * (T)(t + m * NT);
-    if (valid && kok(m)) st32(w[2], row0 + t + m * NT, A[m] + mul_i(kz2 * B[m]));
+    if (valid) st32(w[2], row0 + t + m * NT, A[m] + mul_i(kz2 * B[m]));
   }
+  put_nyq(2, A[P / 2] + mul_i(KN2 * B[P / 2]));
   SLES_FWD(q0, q1)  // N_x, N_y
 #pragma unroll
-  for (int m = 0; m < KH; ++m) {
+  for (int m = 0; m < P / 2; ++m) {
     const T kz2 = T(2) * (T)(t + m * NT);
-    if (valid && kok(m)) {
+    if (valid) {
       st32(w[0], row0 + t + m * NT, A[m] + mul_i(kz2 * t13[m]));
       st32(w[1], row0 + t + m * NT, B[m] + mul_i(kz2 * t23[m]));
     }
   }
+  put_nyq(0, A[P / 2] + mul_i(KN2 * (NQ ? nyq(5) : n13)));
+  put_nyq(1, B[P / 2] + mul_i(KN2 * (NQ ? nyq(7) : n23)));
   T zq[P];
 #pragma unroll
   for (int m = 0; m < P; ++m) zq[m] = T(0);
   SLES_FWD(q3, zq)  // tau_xx
 #pragma unroll
-  for (int m = 0; m < KH; ++m)
-    if (valid && kok(m)) st32(w[3], row0 + t + m * NT, A[m]);
+  for (int m = 0; m < P / 2; ++m)
+    if (valid) st32(w[3], row0 + t + m * NT, A[m]);
+  put_nyq(3, A[P / 2]);
 #undef SLES_FWD
+  if constexpr (NQ) nyquist_store<T, N, LPB, WV>(nq, W, C, 8, pw);
 }
 
 template <typename T, int N, class PL>
@@ -543,33 +740,49 @@ __global__ void __launch_bounds__(ZLines<PL>::THREADS)
   using IFFT = LineFFTP<T, PL, +1, WV, ZPPB<T, N, PL>>;
   using FFT = LineFFTP<T, PL, -1, WV, ZPPB<T, N, PL>>;
   __shared__ cpx<T> lds[LPB * LdsLine<T, N>::STRIDE + ZPPB<T, N, PL>];
+  constexpr int TH = ZLines<PL>::THREADS;
+  constexpr bool NQ = sizeof(T) == 4;  // Nyquist bins staged in LDS (z_row0)
+  __shared__ cpx<T> nq[!NQ ? 1 : 6 * LPB > TH ? 6 * LPB : TH];  // Nyquist bins of the block's lines: 6 inputs, then 3 outputs
   const int line = threadIdx.x / NT, t = threadIdx.x % NT;
   const long row = (long)blockIdx.x * LPB + line;
   const bool valid = row < (long)N * N;
-  const long rv = valid ? row : 0;  // (x, y) line: plane x = rv / N, row y = rv % N
-  const unsigned row0 = (unsigned)((rv / N) * pw + (rv % N) * NH);
+  const unsigned row0 = z_row0<N>(valid ? row : 0, pw);  // (x, y) line: plane x = row / N, row y = row % N
   const long C = N * pw;
   const LdsLine<T, N> map{line * LdsLine<T, N>::STRIDE};
   const TwRegs<T, PL> twr(tw, t);  // read once, used by all 5 transforms
   cpx<T>* w[6];
 #pragma unroll
   for (int i = 0; i < 6; ++i) w[i] = W + i * C;
+  const unsigned nyq_addr = z_nyquist<N>(valid ? row : 0, pw);  // thread 0's Nyquist bin, without NQ
+  auto nyq = [&](int f) -> cpx<T>& { return nq[NQ ? f * LPB + line : 0]; };
+  // Nyquist bin of output field f (thread 0 of each line holds it)
+  auto put_nyq = [&](int f, const cpx<T>& val) {
+    if constexpr (NQ) {
+      if (t == 0) nyq(f) = val;
+    } else {
+      if (valid && t == 0) st32(w[f], nyq_addr, val);
+    }
+  };
   T u0[P], u1[P], u2[P], o0[P], o1[P], o2[P];
   cpx<T> v[P];
-  // pair i+1 is loaded while pair i is transformed
-  PairRaw<T, P> pr;
-  pair_issue<T, N, P, NT>(pr, w[0], w[1], row0, t);
-#define SLES_INV(NA, NB, RA, RB)                                   \
-  pair_finish<T, N, P, NT, WV>(v, pr, false, false, t, lds, map.plain()); \
-  if (NA) pair_issue<T, N, P, NT>(pr, NA, NB, row0, t);            \
-  IFFT::run(v, lds, map, t, twr);                                  \
-  _Pragma("unroll") for (int m = 0; m < P; ++m) {                  \
-    RA[m] = scale * v[m].x;                                        \
-    RB[m] = scale * v[m].y;                                        \
+  // pair i+1 is loaded while pair i is transformed; the Nyquist bins' loads
+  // follow the first pair's, so their latencies overlap
+  PairRaw<T, P, NQ> pr;
+  cpx<T> nqv{};
+  if constexpr (NQ) nqv = nyquist_issue<T, N, LPB, 6, TH>(nq, W, C, pw);
+  pair_issue<T, N, P, NT>(pr, w[0], w[1], row0, nyq_addr, t);
+  if constexpr (NQ) nyquist_stage<T, LPB, 6, TH, WV>(nq, nqv);
+#define SLES_INV(FA, NA, RA, RB)                                                            \
+  pair_finish<T, N, P, NT, WV>(v, pr, nyq(FA), nyq(FA + 1), false, false, t, lds, map.plain()); \
+  if (NA >= 0) pair_issue<T, N, P, NT>(pr, w[NA], w[NA + 1], row0, nyq_addr, t);              \
+  IFFT::run(v, lds, map, t, twr);                                                           \
+  _Pragma("unroll") for (int m = 0; m < P; ++m) {                                           \
+    RA[m] = scale * v[m].x;                                                                 \
+    RB[m] = scale * v[m].y;                                                                 \
   }
-  SLES_INV(w[2], w[3], u0, u1)
-  SLES_INV(w[4], w[5], u2, o0)
-  SLES_INV((cpx<T>*)nullptr, (cpx<T>*)nullptr, o1, o2)
+  SLES_INV(0, 2, u0, u1)
+  SLES_INV(2, 4, u2, o0)
+  SLES_INV(4, -1, o1, o2)
 #undef SLES_INV
   T umax = -INFINITY;
   T n0[P], n1[P], n2[P];
@@ -587,18 +800,22 @@ __global__ void __launch_bounds__(ZLines<PL>::THREADS)
   FFT::run(v, lds, map, t, twr);
   split_pair<T, N, P, NT, WV>(v, lds, map.plain(), t, A, B);
 #pragma unroll
-  for (int m = 0; m < KH; ++m)
-    if (valid && t + m * NT <= N / 2) {
+  for (int m = 0; m < P / 2; ++m)
+    if (valid) {
       st32(w[0], row0 + t + m * NT, A[m]);
       st32(w[1], row0 + t + m * NT, B[m]);
     }
+  put_nyq(0, A[P / 2]);
+  put_nyq(1, B[P / 2]);
 #pragma unroll
   for (int m = 0; m < P; ++m) v[m] = {n2[m], T(0)};
   FFT::run(v, lds, map, t, twr);
   split_pair<T, N, P, NT, WV>(v, lds, map.plain(), t, A, B);
 #pragma unroll
-  for (int m = 0; m < KH; ++m)
-    if (valid && t + m * NT <= N / 2) st32(w[2], row0 + t + m * NT, A[m]);
+  for (int m = 0; m < P / 2; ++m)
+    if (valid) st32(w[2], row0 + t + m * NT, A[m]);
+  put_nyq(2, A[P / 2]);
+  if constexpr (NQ) nyquist_store<T, N, LPB, WV>(nq, W, C, 3, pw);
 }
 
 // ---------------------------------------------------------------------------
@@ -615,14 +832,12 @@ __global__ void __launch_bounds__((WideShape<T, N, PL>::THREADS))
   constexpr int PPB = sizeof(T) == 4 ? N * TK : 0;
   using FFT = LineFFTP<T, PL, -1, false, PPB>;
   __shared__ cpx<T> lds[N * TK + PPB];
-  const Cols<N, TK> col(xcd);
-  const int t = threadIdx.x / TK;
+  const Cols<N, TK> col(xcd, NT, y_nyquist_tile<N, TK>(xcd));
+  const int t = col.t;
   const TwRegs<T, PL> twr(tw, t);  // read once per block, used by every transform
   const LdsColumn<TK> map{col.c};
-  const long ix = col.g / NH, iz = col.g % NH;
-  const eidx row0 = (eidx)ix * pw + (eidx)t * NH + (eidx)iz;
-  auto at = [&](int m) { return row0 + (eidx)(m * NT * NH); };
   const long C = N * pw;
+  y_rows<N, TK, NT>(col.g, col.g0, pw, t, [&](auto at) {
   cpx<T> v[P], acc[P];
   auto load = [&](int f) {
 #pragma unroll
@@ -673,6 +888,7 @@ __global__ void __launch_bounds__((WideShape<T, N, PL>::THREADS))
       store(f, v);
     }
   }
+  });
 }
 
 // ---------------------------------------------------------------------------
@@ -812,8 +1028,8 @@ using XShape = StridedShape<T, N, PL, (sizeof(T) == 4 && PL::P == 4) ? 1024 : 51
 // stage runs that pass's x-inverse on the new U_hat).
 template <typename T, int N, int LES, class PL, bool FUSE = false, int FLES = LES>
 __global__ void __launch_bounds__((XShape<T, N, PL>::THREADS))
-    k_xfwd(cpx<T>* __restrict__ W, long pw, cpx<T>* uh, cpx<T>* uh0, cpx<T>* uh1, cpx<T>* aux, long ps, StepArgs a,
-           const cpx<T>* __restrict__ tw, int xcd, Members mb) {
+    k_xfwd(cpx<T>* __restrict__ W, long pw, cpx<T>* uh, cpx<T>* uh0, cpx<T>* uh1, cpx<T>* aux, long ps, int ssplit,
+           StepArgs a, const cpx<T>* __restrict__ tw, int xcd, Members mb) {
   W = member_ptr(W, mb.work);
   uh = member_ptr(uh, mb.state);
   if (uh0) uh0 = member_ptr(uh0, mb.state);
@@ -838,7 +1054,8 @@ __global__ void __launch_bounds__((XShape<T, N, PL>::THREADS))
   const int t = threadIdx.x / TK;
   const TwRegs<T, PL> twr(tw, t);  // read once per block, used by every transform
   const LdsColumn<TK> map{col.c};
-  const XRows rw(t, NT, pw, col.g), rs(t, NT, ps, col.g);  // W rows, RK-state rows
+  const XCol<N> xc(col.g, ssplit);
+  const XRows rw(t, NT, pw, xc.jw), rs(t, NT, ps, xc.js);  // W rows, RK-state rows
   const long C = N * pw, CS = N * ps;
   // a0..a2: S_i = F_x(W_i) (+ 2i Kx F_x(tau_xi) with LES).  Named arrays and
   // a rolled field loop with uniform branches keep them in registers.
@@ -884,7 +1101,7 @@ __global__ void __launch_bounds__((XShape<T, N, PL>::THREADS))
       }
     }
   }
-  const int ky = kfreq((int)(col.g / NH), N), kz = (int)(col.g % NH);
+  const int ky = kfreq(xc.iy, N), kz = xc.kz;
   cpx<T> y0[P], y1[P], y2[P];
 #pragma unroll
   for (int m = 0; m < P; ++m) {
@@ -1103,8 +1320,8 @@ struct Fused {
 
   // skip_xinv: W0-5 already hold the x-inverse output (fused into the previous x-forward)
   // nm members (grid rows) at the strides of mb; ps: kx-plane stride of the RK-state arrays
-  static int stage(const sles_plan_s* p, int les, T coeff, cpx<T>* uh, long ps, cpx<T>* W, double* part,
-                   hipStream_t s, bool skip_xinv, const Members& mb, int nm) {
+  static int stage(const sles_plan_s* p, int les, T coeff, cpx<T>* uh, long ps, int ssplit, cpx<T>* W,
+                   double* part, hipStream_t s, bool skip_xinv, const Members& mb, int nm) {
     const long pw = plane_w(p);
     const cpx<T>* tw = (const cpx<T>*)p->tw;
     const int xcd = p->tune_xcd;
@@ -1115,7 +1332,7 @@ struct Fused {
     if (les) {
       if (!skip_xinv) {
         ProfScope ps_(PK_XINV, s);
-        hipLaunchKernelGGL((k_xinv<T, N, 1, PS>), gs, bs, 0, s, uh, ps, W, pw, tw, xcd, mb);
+        hipLaunchKernelGGL((k_xinv<T, N, 1, PS>), gs, bs, 0, s, uh, ps, ssplit, W, pw, tw, xcd, mb);
       }
       { ProfScope ps_(PK_YINV, s); hipLaunchKernelGGL((k_yinv<T, N, 1, PS>), gs, bs, 0, s, W, pw, tw, xcd, mb); }
       ProfScope ps_(PK_Z, s);
@@ -1126,7 +1343,7 @@ struct Fused {
     } else {
       if (!skip_xinv) {
         ProfScope ps_(PK_XINV, s);
-        hipLaunchKernelGGL((k_xinv<T, N, 0, PS>), gs, bs, 0, s, uh, ps, W, pw, tw, xcd, mb);
+        hipLaunchKernelGGL((k_xinv<T, N, 0, PS>), gs, bs, 0, s, uh, ps, ssplit, W, pw, tw, xcd, mb);
       }
       { ProfScope ps_(PK_YINV, s); hipLaunchKernelGGL((k_yinv<T, N, 0, PS>), gs, bs, 0, s, W, pw, tw, xcd, mb); }
       ProfScope ps_(PK_Z, s);
@@ -1138,8 +1355,8 @@ struct Fused {
   // fuse_next: also run the next stage's x-inverse (RK4 stages 0-2)
   // fuse_phys (last RK4 stage): also run the physical-output pass's x-inverse
   static int finish(const sles_plan_s* p, int les, cpx<T>* uh, cpx<T>* uh0, cpx<T>* uh1, cpx<T>* aux, long ps,
-                    cpx<T>* W, const StepArgs& a, hipStream_t s, bool fuse_next, bool fuse_phys, const Members& mb,
-                    int nm) {
+                    int ssplit, cpx<T>* W, const StepArgs& a, hipStream_t s, bool fuse_next, bool fuse_phys,
+                    const Members& mb, int nm) {
     const long pw = plane_w(p);
     const cpx<T>* tw = (const cpx<T>*)p->tw;
     const int xcd = p->tune_xcd;
@@ -1151,56 +1368,57 @@ struct Fused {
       ProfScope ps_(PK_XFWD, s);
       if constexpr (FUSE_X) {
         if (fuse_next) {
-          hipLaunchKernelGGL((k_xfwd<T, N, 1, PX, true>), gx, bx, 0, s, W, pw, uh, uh0, uh1, aux, ps, a, tw, xcd, mb);
+          hipLaunchKernelGGL((k_xfwd<T, N, 1, PX, true>), gx, bx, 0, s, W, pw, uh, uh0, uh1, aux, ps, ssplit, a, tw, xcd, mb);
           return check_launch("fused stage (forward)");
         }
         if (fuse_phys) {
-          hipLaunchKernelGGL((k_xfwd<T, N, 1, PX, true, 0>), gx, bx, 0, s, W, pw, uh, uh0, uh1, aux, ps, a, tw, xcd,
+          hipLaunchKernelGGL((k_xfwd<T, N, 1, PX, true, 0>), gx, bx, 0, s, W, pw, uh, uh0, uh1, aux, ps, ssplit, a, tw, xcd,
                              mb);
           return check_launch("fused stage (forward)");
         }
       }
-      hipLaunchKernelGGL((k_xfwd<T, N, 1, PX>), gx, bx, 0, s, W, pw, uh, uh0, uh1, aux, ps, a, tw, xcd, mb);
+      hipLaunchKernelGGL((k_xfwd<T, N, 1, PX>), gx, bx, 0, s, W, pw, uh, uh0, uh1, aux, ps, ssplit, a, tw, xcd, mb);
     } else {
       { ProfScope ps_(PK_YFWD, s); hipLaunchKernelGGL((k_yfwd<T, N, 0, PS>), gs, bs, 0, s, W, pw, tw, xcd, mb); }
       ProfScope ps_(PK_XFWD, s);
       if constexpr (FUSE_X)
         if (fuse_next || fuse_phys) {  // the same output layout
-          hipLaunchKernelGGL((k_xfwd<T, N, 0, PX, true>), gx, bx, 0, s, W, pw, uh, uh0, uh1, aux, ps, a, tw, xcd, mb);
+          hipLaunchKernelGGL((k_xfwd<T, N, 0, PX, true>), gx, bx, 0, s, W, pw, uh, uh0, uh1, aux, ps, ssplit, a, tw, xcd, mb);
           return check_launch("fused stage (forward)");
         }
-      hipLaunchKernelGGL((k_xfwd<T, N, 0, PX>), gx, bx, 0, s, W, pw, uh, uh0, uh1, aux, ps, a, tw, xcd, mb);
+      hipLaunchKernelGGL((k_xfwd<T, N, 0, PX>), gx, bx, 0, s, W, pw, uh, uh0, uh1, aux, ps, ssplit, a, tw, xcd, mb);
     }
     return check_launch("fused stage (forward)");
   }
 
   // skip_xinv: W0-4 already hold the x-inverse output (fused into the last x-forward)
-  static int physical(const sles_plan_s* p, cpx<T>* uh, long ps, cpx<T>* W, void* u_out, void* curl_out,
-                      hipStream_t s, bool skip_xinv = false) {
+  static int physical(const sles_plan_s* p, cpx<T>* uh, long ps, int ssplit, cpx<T>* W, void* u_out,
+                      void* curl_out, hipStream_t s, bool skip_xinv = false) {
     const long pw = plane_w(p);
     const cpx<T>* tw = (const cpx<T>*)p->tw;
     const int xcd = p->tune_xcd;
     constexpr long NCOL = (long)N * (N / 2 + 1);  // columns of every strided pass
     const dim3 gs((unsigned)((NCOL + SS::TK - 1) / SS::TK)), bs(SS::THREADS);
     ProfScope ps_(PK_PHYS, s);
-    if (!skip_xinv) hipLaunchKernelGGL((k_xinv<T, N, 0, PS>), gs, bs, 0, s, uh, ps, W, pw, tw, xcd, Members{});
+    if (!skip_xinv) hipLaunchKernelGGL((k_xinv<T, N, 0, PS>), gs, bs, 0, s, uh, ps, ssplit, W, pw, tw, xcd, Members{});
     hipLaunchKernelGGL((k_yinv<T, N, 0, PS>), gs, bs, 0, s, W, pw, tw, xcd, Members{});
     if (check_launch("fused physical")) return -1;
     const double sc = 1.0 / ((double)N * N * N);
-    if (u_out && z_c2r_rows(p, W, pw, u_out, 3L * N * N, sc, s)) return -1;
-    if (curl_out && z_c2r_rows(p, W + 3 * N * pw, pw, curl_out, 3L * N * N, sc, s)) return -1;
+    if (u_out && z_c2r_rows(p, W, pw, 1, u_out, 3L * N * N, sc, s)) return -1;
+    if (curl_out && z_c2r_rows(p, W + 3 * N * pw, pw, 1, curl_out, 3L * N * N, sc, s)) return -1;
     return 0;
   }
 };
 
 // One fused step (or RHS) of `scheme` over the RK-state arrays uh, uh0, uh1,
-// aux, whose kx planes lie ps elements apart (the reference layout:
-// N (N/2+1); the driver's padded copies: plane_w); the workspace W holds
-// the padded fields and the CFL partials (work_fields_bytes).
+// aux, whose kx planes lie ps elements apart and are arranged as the
+// reference's (ssplit = 0, ps = N (N/2+1)) or as the workspace's (ssplit = 1,
+// the drivers' copies, ps = plane_w); the workspace W holds the padded
+// fields and the CFL partials (work_fields_bytes).
 template <typename T, int N, class PS, class PX, class PZ>
 int step_impl(const sles_plan_s* p, int scheme, const sles_step_params* prm, void* uh_, void* uh0_, void* uh1_,
-              void* aux_, long ps, void* u_out, void* curl_out, double* scalars, void* work, hipStream_t s,
-              void* stages, const DriveArgs* drive, const Batch* batch) {
+              void* aux_, long ps, int ssplit, void* u_out, void* curl_out, double* scalars, void* work,
+              hipStream_t s, void* stages, const DriveArgs* drive, const Batch* batch) {
   using F = Fused<T, N, PS, PX, PZ>;
   const int nm = batch ? batch->count : 1;
   const Members mb = batch ? batch->mb : Members{0, 0, 0, nullptr};
@@ -1240,7 +1458,7 @@ int step_impl(const sles_plan_s* p, int scheme, const sles_step_params* prm, voi
     double* pp = (ddt && st == 0) ? part : nullptr;
     const bool fuse = F::FUSE_X && p->tune_fuse_x && scheme == SLES_SCHEME_RK4;
     const bool fused_in = fuse && st > 0;  // x-inverse ran in the previous x-forward
-    if (F::stage(p, les, coeff, uh, ps, W, pp, s, fused_in, mb, nm)) return -1;
+    if (F::stage(p, les, coeff, uh, ps, ssplit, W, pp, s, fused_in, mb, nm)) return -1;
     if (pp) {
       // two levels: up to 64 chunks of the per-block maxima, then one block
       // (the partials area holds N^2 + 64 doubles; the z grid is at most N^2)
@@ -1258,13 +1476,13 @@ int step_impl(const sles_plan_s* p, int scheme, const sles_step_params* prm, voi
                            scalars);
       if (check_launch("cfl")) return -1;
     }
-    if (F::finish(p, les, uh, uh0, uh1, aux, ps, W, a, s, fuse && st < 3, fuse && st == 3 && (u_out || curl_out),
-                  mb, nm))
+    if (F::finish(p, les, uh, uh0, uh1, aux, ps, ssplit, W, a, s, fuse && st < 3,
+                  fuse && st == 3 && (u_out || curl_out), mb, nm))
       return -1;
   }
   // the last fused RK4 x-forward already ran the physical pass's x-inverse
   const bool phys_fused = F::FUSE_X && p->tune_fuse_x && scheme == SLES_SCHEME_RK4;
-  if (u_out || curl_out) return F::physical(p, uh, ps, W, u_out, curl_out, s, phys_fused);
+  if (u_out || curl_out) return F::physical(p, uh, ps, ssplit, W, u_out, curl_out, s, phys_fused);
   return 0;
 }
 
@@ -1280,16 +1498,16 @@ struct HasP4 {
 
 template <typename T, int N>
 int step_t(const sles_plan_s* p, int scheme, const sles_step_params* prm, void* uh, void* uh0, void* uh1, void* aux,
-           long ps, void* u_out, void* curl_out, double* scalars, void* work, hipStream_t s, void* stages,
-           const DriveArgs* drive, const Batch* batch) {
+           long ps, int ssplit, void* u_out, void* curl_out, double* scalars, void* work, hipStream_t s,
+           void* stages, const DriveArgs* drive, const Batch* batch) {
   using D = LinePlan<N>;
   if constexpr (HasP4<N>::value) {
     using Q = LinePlan4<N>;
     const int sel = (p->tune_strided ? 1 : 0) | (p->tune_xfwd ? 2 : 0) | (p->tune_z ? 4 : 0);
 #define SLES_SEL(K, A, B, Cz) \
   case K:                     \
-    return step_impl<T, N, A, B, Cz>(p, scheme, prm, uh, uh0, uh1, aux, ps, u_out, curl_out, scalars, work, s, \
-                                     stages, drive, batch);
+    return step_impl<T, N, A, B, Cz>(p, scheme, prm, uh, uh0, uh1, aux, ps, ssplit, u_out, curl_out, scalars, \
+                                     work, s, stages, drive, batch);
     switch (sel) {
       SLES_SEL(0, D, D, D)
       SLES_SEL(1, Q, D, D)
@@ -1302,17 +1520,19 @@ int step_t(const sles_plan_s* p, int scheme, const sles_step_params* prm, void* 
     }
 #undef SLES_SEL
   }
-  return step_impl<T, N, D, D, D>(p, scheme, prm, uh, uh0, uh1, aux, ps, u_out, curl_out, scalars, work, s,
-                                  stages, drive, batch);
+  return step_impl<T, N, D, D, D>(p, scheme, prm, uh, uh0, uh1, aux, ps, ssplit, u_out, curl_out, scalars, work,
+                                  s, stages, drive, batch);
 }
 
 // Per-precision instantiation (sles_rhs_f64.hip / sles_rhs_f32.hip).
-// ps: kx-plane stride (elements) of the RK-state arrays uh, uh0, uh1, aux, stages.
+// ps, ssplit: kx-plane stride (elements) and arrangement (0: the
+// reference's, 1: the workspace's) of the RK-state arrays uh, uh0, uh1, aux,
+// stages.
 int step_f64(const sles_plan_s* p, int scheme, const sles_step_params* prm, void* uh, void* uh0, void* uh1,
-             void* aux, long ps, void* u_out, void* curl_out, double* scalars, void* work, hipStream_t s,
-             void* stages, const DriveArgs* drive = nullptr, const Batch* batch = nullptr);
+             void* aux, long ps, int ssplit, void* u_out, void* curl_out, double* scalars, void* work,
+             hipStream_t s, void* stages, const DriveArgs* drive = nullptr, const Batch* batch = nullptr);
 int step_f32(const sles_plan_s* p, int scheme, const sles_step_params* prm, void* uh, void* uh0, void* uh1,
-             void* aux, long ps, void* u_out, void* curl_out, double* scalars, void* work, hipStream_t s,
-             void* stages, const DriveArgs* drive = nullptr, const Batch* batch = nullptr);
+             void* aux, long ps, int ssplit, void* u_out, void* curl_out, double* scalars, void* work,
+             hipStream_t s, void* stages, const DriveArgs* drive = nullptr, const Batch* batch = nullptr);
 
 }  // namespace sles

@@ -71,14 +71,18 @@ size_t steps_slot_offset(const sles_plan_s* p, int les_model) {
 size_t mirror_offset(const sles_plan_s* p, int les_model) { return steps_slot_offset(p, les_model) + 256; }
 constexpr int kMirrorFields = 12;
 
-// Copies 3 spectral fields of N kx planes (plane_bytes each) from src to
-// dst, planes sp / dp bytes apart (fields N planes apart); blockIdx.z is the
-// member (sm / dm bytes apart).  With `steps` set, member m copies only if
-// its driver state counts a step taken since `before` saved the count (so a
-// call that took no step leaves the destination as it was).
-__global__ void k_relayout(const char* __restrict__ src, long sp, long sm, char* __restrict__ dst, long dp, long dm,
-                           long plane_bytes, int n, const double* steps, const double* before, long drive_stride,
-                           long before_stride) {
+// Copies 3 spectral fields of N kx planes between the caller's arrays
+// (reference layout: planes N (N/2+1) values apart, rows of N/2+1; members
+// ref_m bytes apart) and the workspace's copies (planes pw values apart,
+// split arrangement: rows of N/2, then the Nyquist column; members wrk_m
+// bytes apart); fields lie N planes apart in both.  to_split: caller ->
+// workspace, else workspace -> caller.  blockIdx.z is the member.  With
+// `steps` set, member m copies only if its driver state counts a step taken
+// since `before` saved the count (so a call that took no step leaves the
+// caller's array as it was).
+template <class E>
+__global__ void k_relayout(E* ref, long ref_m, E* wrk, long pw, long wrk_m, int n, int to_split, const double* steps,
+                           const double* before, long drive_stride, long before_stride) {
   const long m = blockIdx.z;
   if (steps) {
     const double now = *(const double*)((const char*)steps + m * drive_stride);
@@ -86,11 +90,19 @@ __global__ void k_relayout(const char* __restrict__ src, long sp, long sm, char*
     if (now == then) return;
   }
   const long plane = blockIdx.y;  // field * n + kx plane
-  const char* s = src + m * sm + plane * sp;
-  char* d = dst + m * dm + plane * dp;
-  const long stride = (long)gridDim.x * blockDim.x * 16;
-  for (long o = ((long)blockIdx.x * blockDim.x + threadIdx.x) * 16; o < plane_bytes; o += stride)
-    *(uint4*)(d + o) = *(const uint4*)(s + o);
+  const int nr = n / 2, nh = n / 2 + 1;
+  const long main = (long)n * nr, total = (long)n * nh;
+  E* R = (E*)((char*)ref + m * ref_m) + plane * total;
+  E* Wk = (E*)((char*)wrk + m * wrk_m) + plane * pw;
+  const long stride = (long)gridDim.x * blockDim.x;
+  for (long j = (long)blockIdx.x * blockDim.x + threadIdx.x; j < total; j += stride) {
+    // j: column of the split plane; r: the same value in the caller's plane
+    const long r = j < main ? (j / nr) * nh + j % nr : (j - main) * nh + nr;
+    if (to_split)
+      Wk[j] = R[r];
+    else
+      R[r] = Wk[j];
+  }
 }
 
 // before[m] = the steps count of member m's driver state, m < members
@@ -101,13 +113,18 @@ __global__ void k_save_steps(const double* steps, long drive_stride, double* bef
     *(double*)((char*)before + m * before_stride) = *(const double*)((const char*)steps + m * drive_stride);
 }
 
-int relayout(const sles_plan_s* p, const void* src, long sp, long sm, void* dst, long dp, long dm, int members,
+// k_relayout over `members` members of one array (3 fields).
+int relayout(const sles_plan_s* p, int to_split, void* ref, long ref_m, void* wrk, long wrk_m, int members,
              const double* steps, const double* before, long drive_stride, long before_stride, hipStream_t s) {
-  const long plane_bytes = ncplx(p) / p->n * cplx_bytes(p);
-  const unsigned chunks = (unsigned)((plane_bytes / 16 + 1023) / 1024);  // 256 threads x 4 copies each
-  hipLaunchKernelGGL(k_relayout, dim3(chunks, (unsigned)(3 * p->n), (unsigned)members), dim3(256), 0, s,
-                     (const char*)src, sp, sm, (char*)dst, dp, dm, plane_bytes, p->n, steps, before, drive_stride,
-                     before_stride);
+  const long per_plane = ncplx(p) / p->n;
+  const dim3 grid((unsigned)((per_plane + 1023) / 1024), (unsigned)(3 * p->n), (unsigned)members);
+  const long pw = plane_w(p);
+  if (p->precision == SLES_FP64)
+    hipLaunchKernelGGL(k_relayout<uint4>, grid, dim3(256), 0, s, (uint4*)ref, ref_m, (uint4*)wrk, pw, wrk_m, p->n,
+                       to_split, steps, before, drive_stride, before_stride);
+  else
+    hipLaunchKernelGGL(k_relayout<uint2>, grid, dim3(256), 0, s, (uint2*)ref, ref_m, (uint2*)wrk, pw, wrk_m, p->n,
+                       to_split, steps, before, drive_stride, before_stride);
   return check_launch("state relayout");
 }
 
@@ -118,6 +135,7 @@ int relayout(const sles_plan_s* p, const void* src, long sp, long sm, void* dst,
 struct DriveState {
   void *uh, *uh0, *uh1, *aux;
   long ps;          // kx-plane stride (elements)
+  int split;        // arrangement: 0 the caller's, 1 the workspace's
   long mstride;     // bytes between members' arrays
 };
 
@@ -131,7 +149,7 @@ int drive_enter(const sles_plan_s* p, int scheme, int les, void* uh, void* uh0, 
                 long sbytes, double* state, long dstride, void* work, long wstride, hipStream_t s, DriveState* ds) {
   const long dense = ncplx(p) / p->n;
   if (p->tune_pad == 0) {
-    *ds = DriveState{uh, uh0, uh1, aux, dense, sbytes};
+    *ds = DriveState{uh, uh0, uh1, aux, dense, 0, sbytes};
     return 0;
   }
   // copies of the arrays the scheme touches: U_hat0 (RK4, RK3), U_hat1 (RK4), aux
@@ -140,15 +158,13 @@ int drive_enter(const sles_plan_s* p, int scheme, int les, void* uh, void* uh0, 
   const bool rk = scheme == SLES_SCHEME_RK4 || scheme == SLES_SCHEME_RK3;
   const bool rk4 = scheme == SLES_SCHEME_RK4;
   *ds = DriveState{m, rk && uh0 ? m + 3 * fbytes : nullptr, rk4 && uh1 ? m + 6 * fbytes : nullptr,
-                   aux ? m + 9 * fbytes : nullptr, plane_w(p), wstride};
-  const long pb = dense * cplx_bytes(p), pwb = plane_w(p) * cplx_bytes(p);
+                   aux ? m + 9 * fbytes : nullptr, plane_w(p), 1, wstride};
   double* before = (double*)((char*)work + steps_slot_offset(p, les));
   hipLaunchKernelGGL(k_save_steps, dim3((unsigned)((members + 255) / 256)), dim3(256), 0, s,
                      state + SLES_DRIVE_STATE_STEPS, dstride, before, wstride, members);
   if (check_launch("driver steps count")) return -1;
-  if (relayout(p, uh, pb, sbytes, ds->uh, pwb, wstride, members, nullptr, nullptr, 0, 0, s)) return -1;
-  if (scheme == SLES_SCHEME_ETD2 && relayout(p, aux, pb, sbytes, ds->aux, pwb, wstride, members, nullptr, nullptr,
-                                             0, 0, s))
+  if (relayout(p, 1, uh, sbytes, ds->uh, wstride, members, nullptr, nullptr, 0, 0, s)) return -1;
+  if (scheme == SLES_SCHEME_ETD2 && relayout(p, 1, aux, sbytes, ds->aux, wstride, members, nullptr, nullptr, 0, 0, s))
     return -1;
   return 0;
 }
@@ -161,18 +177,15 @@ int drive_exit(const sles_plan_s* p, int scheme, int les, void* uh, void* uh0, v
                long sbytes, double* state, long dstride, void* work, long wstride, hipStream_t s,
                const DriveState& ds) {
   if (p->tune_pad == 0) return 0;
-  const long pb = ncplx(p) / p->n * cplx_bytes(p), pwb = plane_w(p) * cplx_bytes(p);
   const double* steps = state + SLES_DRIVE_STATE_STEPS;
   const double* before = (const double*)((char*)work + steps_slot_offset(p, les));
-  if (relayout(p, ds.uh, pwb, wstride, uh, pb, sbytes, members, nullptr, nullptr, 0, 0, s)) return -1;
-  if (ds.uh0 && relayout(p, ds.uh0, pwb, wstride, uh0, pb, sbytes, members, steps, before, dstride, wstride, s))
-    return -1;
-  if (ds.uh1 && relayout(p, ds.uh1, pwb, wstride, uh1, pb, sbytes, members, steps, before, dstride, wstride, s))
-    return -1;
+  if (relayout(p, 0, uh, sbytes, ds.uh, wstride, members, nullptr, nullptr, 0, 0, s)) return -1;
+  if (ds.uh0 && relayout(p, 0, uh0, sbytes, ds.uh0, wstride, members, steps, before, dstride, wstride, s)) return -1;
+  if (ds.uh1 && relayout(p, 0, uh1, sbytes, ds.uh1, wstride, members, steps, before, dstride, wstride, s)) return -1;
   if (ds.aux) {
     const bool always = scheme == SLES_SCHEME_ETD2;
-    if (relayout(p, ds.aux, pwb, wstride, aux, pb, sbytes, members, always ? nullptr : steps,
-                 always ? nullptr : before, dstride, wstride, s))
+    if (relayout(p, 0, aux, sbytes, ds.aux, wstride, members, always ? nullptr : steps, always ? nullptr : before,
+                 dstride, wstride, s))
       return -1;
   }
   return 0;
@@ -207,8 +220,9 @@ int sles_step(sles_plan p, int scheme, const sles_step_params* prm, void* u_hat,
   if (!sles_supported_n(p->n)) return fail("sles_step: unsupported grid size");
   const long ps = ncplx(p) / p->n;  // the caller's arrays: reference layout
   return p->precision == SLES_FP64
-             ? step_f64(p, scheme, prm, u_hat, u_hat0, u_hat1, aux, ps, u_out, curl_out, scalars, work, s, nullptr)
-             : step_f32(p, scheme, prm, u_hat, u_hat0, u_hat1, aux, ps, u_out, curl_out, scalars, work, s, nullptr);
+             ? step_f64(p, scheme, prm, u_hat, u_hat0, u_hat1, aux, ps, 0, u_out, curl_out, scalars, work, s, nullptr)
+             : step_f32(p, scheme, prm, u_hat, u_hat0, u_hat1, aux, ps, 0, u_out, curl_out, scalars, work, s,
+                        nullptr);
 }
 
 int sles_step_noalloc(sles_plan p, int scheme, const sles_step_params* prm, void* u_hat, void* u_hat0,
@@ -258,8 +272,10 @@ int sles_step_drive(sles_plan p, int scheme, const sles_step_params* prm, double
     void* co = last ? curl_out : nullptr;
     const int rc =
         p->precision == SLES_FP64
-            ? step_f64(p, scheme, prm, ds.uh, ds.uh0, ds.uh1, ds.aux, ds.ps, uo, co, nullptr, work, s, nullptr, &drv)
-            : step_f32(p, scheme, prm, ds.uh, ds.uh0, ds.uh1, ds.aux, ds.ps, uo, co, nullptr, work, s, nullptr, &drv);
+            ? step_f64(p, scheme, prm, ds.uh, ds.uh0, ds.uh1, ds.aux, ds.ps, ds.split, uo, co, nullptr, work, s,
+                       nullptr, &drv)
+            : step_f32(p, scheme, prm, ds.uh, ds.uh0, ds.uh1, ds.aux, ds.ps, ds.split, uo, co, nullptr, work, s,
+                       nullptr, &drv);
     if (rc) return rc;
   }
   return drive_exit(p, scheme, prm->les_model, u_hat, u_hat0, u_hat1, aux, 1, sbytes, state, 0, work, 0, s, ds);
@@ -310,10 +326,10 @@ int sles_step_drive_batch(sles_plan p, int scheme, const sles_step_params* prm, 
   const DriveArgs drv{state, cfl_first};
   for (int i = 0; i < nsteps; ++i) {
     const int rc = p->precision == SLES_FP64
-                       ? step_f64(p, scheme, prm, ds.uh, ds.uh0, ds.uh1, ds.aux, ds.ps, nullptr, nullptr, nullptr,
-                                  work, s, nullptr, &drv, &b)
-                       : step_f32(p, scheme, prm, ds.uh, ds.uh0, ds.uh1, ds.aux, ds.ps, nullptr, nullptr, nullptr,
-                                  work, s, nullptr, &drv, &b);
+                       ? step_f64(p, scheme, prm, ds.uh, ds.uh0, ds.uh1, ds.aux, ds.ps, ds.split, nullptr, nullptr,
+                                  nullptr, work, s, nullptr, &drv, &b)
+                       : step_f32(p, scheme, prm, ds.uh, ds.uh0, ds.uh1, ds.aux, ds.ps, ds.split, nullptr, nullptr,
+                                  nullptr, work, s, nullptr, &drv, &b);
     if (rc) return rc;
   }
   return drive_exit(p, scheme, prm->les_model, u_hat, u_hat0, u_hat1, aux, members, sbytes, state, dstride, work,
@@ -330,10 +346,10 @@ int sles_rk4_stage_states(sles_plan p, const sles_step_params* prm, void* u_hat,
   hipStream_t s = (hipStream_t)stream;
   const long ps = ncplx(p) / p->n;  // the caller's arrays: reference layout
   return p->precision == SLES_FP64
-             ? step_f64(p, SLES_SCHEME_RK4, prm, u_hat, u_hat0, u_hat1, nullptr, ps, nullptr, nullptr, nullptr, work,
-                        s, stages)
-             : step_f32(p, SLES_SCHEME_RK4, prm, u_hat, u_hat0, u_hat1, nullptr, ps, nullptr, nullptr, nullptr, work,
-                        s, stages);
+             ? step_f64(p, SLES_SCHEME_RK4, prm, u_hat, u_hat0, u_hat1, nullptr, ps, 0, nullptr, nullptr, nullptr,
+                        work, s, stages)
+             : step_f32(p, SLES_SCHEME_RK4, prm, u_hat, u_hat0, u_hat1, nullptr, ps, 0, nullptr, nullptr, nullptr,
+                        work, s, stages);
 }
 
 int sles_profile_enable(int on) {

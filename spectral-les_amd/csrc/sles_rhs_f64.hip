@@ -4,12 +4,12 @@
 namespace sles {
 
 int step_f64(const sles_plan_s* p, int scheme, const sles_step_params* prm, void* uh, void* uh0, void* uh1,
-             void* aux, long ps, void* u_out, void* curl_out, double* scalars, void* work, hipStream_t s,
-             void* stages, const DriveArgs* drive, const Batch* batch) {
+             void* aux, long ps, int ssplit, void* u_out, void* curl_out, double* scalars, void* work,
+             hipStream_t s, void* stages, const DriveArgs* drive, const Batch* batch) {
 #define SLES_CASE(NN) \
   case NN:            \
-    return step_t<double, NN>(p, scheme, prm, uh, uh0, uh1, aux, ps, u_out, curl_out, scalars, work, s, stages, drive, \
-                         batch);
+    return step_t<double, NN>(p, scheme, prm, uh, uh0, uh1, aux, ps, ssplit, u_out, curl_out, scalars, work, s, stages, \
+                         drive, batch);
   switch (p->n) {
     SLES_FOR_EACH_N(SLES_CASE)
     default:

@@ -1,9 +1,10 @@
 // Measurement kernel: the memory access pattern of the strided passes alone.
 // A tile of TK consecutive columns of each (N rows) x (M = N (N/2+1)
-// columns) spectral field; thread (c = tid % TK, t = tid / TK) reads rows
-// t + m NT (m < N/NT) of every input field, sums them, and writes the sum to
-// every output field: the loads and stores of an x pass with no transform.
-// Its bandwidth is the ceiling of a pass with that tile shape.
+// columns) spectral field, rows `plane` elements apart (the plan's workspace
+// layout: M + the SLES_TUNE_PAD padding); thread (c = tid % TK, t = tid / TK)
+// reads rows t + m NT (m < N/NT) of every input field, sums them, and writes
+// the sum to every output field: the loads and stores of an x pass with no
+// transform.  Its bandwidth is the ceiling of a pass with that tile shape.
 #include <hip/hip_runtime.h>
 
 #include "sles.h"
@@ -13,9 +14,9 @@ namespace sles {
 
 template <typename T, int N, int TK, int NT>
 __global__ void __launch_bounds__(TK* NT)
-    k_stream(const cpx<T>* __restrict__ in, int nin, cpx<T>* __restrict__ out, int nout, long C, int xcd) {
+    k_stream(const cpx<T>* __restrict__ in, int nin, cpx<T>* __restrict__ out, int nout, long plane, int xcd) {
   constexpr int P = N / NT;
-  constexpr long M = (long)N * (N / 2 + 1);
+  const long C = N * plane;
   const int c = threadIdx.x % TK, t = threadIdx.x / TK;
   const long g = tile_id(xcd) * TK + c;
   cpx<T> acc[P];
@@ -24,20 +25,19 @@ __global__ void __launch_bounds__(TK* NT)
 #pragma unroll 1
   for (int f = 0; f < nin; ++f)
 #pragma unroll
-    for (int m = 0; m < P; ++m) acc[m] = acc[m] + in[f * C + (long)(t + m * NT) * M + g];
+    for (int m = 0; m < P; ++m) acc[m] = acc[m] + in[f * C + (long)(t + m * NT) * plane + g];
 #pragma unroll 1
   for (int f = 0; f < nout; ++f)
 #pragma unroll
-    for (int m = 0; m < P; ++m) out[f * C + (long)(t + m * NT) * M + g] = acc[m];
+    for (int m = 0; m < P; ++m) out[f * C + (long)(t + m * NT) * plane + g] = acc[m];
 }
 
 template <typename T, int N, int TK, int NT>
 static int launch_stream(const sles_plan_s* p, const void* in, int nin, void* out, int nout, hipStream_t s) {
   static_assert(N % NT == 0 && ((long)N * (N / 2 + 1)) % TK == 0, "tile must divide the field");
-  const long C = ncplx(p);
   const unsigned grid = (unsigned)((long)N * (N / 2 + 1) / TK);
   hipLaunchKernelGGL((k_stream<T, N, TK, NT>), dim3(grid), dim3(TK * NT), 0, s, (const cpx<T>*)in, nin,
-                     (cpx<T>*)out, nout, C, p->tune_xcd);
+                     (cpx<T>*)out, nout, plane_w(p), p->tune_xcd);
   return check_launch("stream probe");
 }
 

@@ -365,23 +365,30 @@ def ensemble_adjoint(groups=((64, 8), (96, 8), (128, 8)), steps=200, checkpoint_
     return out
 
 
-def stream_ceiling(N=256, precision="fp64", mixes=((15, 6),), tiles=(0, 1, 2), xcd=1, reps=10, verbose=True):
+def stream_ceiling(N=256, precision="fp64", mixes=((15, 6),), tiles=(0, 1, 2), xcd=1, reps=10, verbose=True,
+                   pad=None):
     """Bandwidth of the strided x-pass access pattern with no arithmetic
     (sles_stream_probe): for each tile shape and (fields read, fields
     written) mix, the algorithmic bytes over the mean launch time.  The
-    ceiling of a pass with that tile shape and traffic mix on this device."""
+    ceiling of a pass with that tile shape and traffic mix on this device.
+    The fields have the fused step's workspace layout: kx planes padded by
+    ``pad`` 128-byte lines (None: the plan's default)."""
     from . import _lib
 
     prec = _lib.SLES_FP64 if precision == "fp64" else _lib.SLES_FP32
     cdt = torch.complex128 if precision == "fp64" else torch.complex64
     plan = _lib.Plan(N, prec, torch.device("cuda"))
     plan.set_tuning(xcd=xcd)
+    if pad is not None:
+        plan.set_tuning(pad=pad)
+    pad = plan.get_tuning()["pad"]
+    esz = 16 if precision == "fp64" else 8
+    plane = N * (N // 2 + 1) + pad * 128 // esz
     nin = max(r for r, _ in mixes)
     nout = max(w for _, w in mixes)
-    shape = (N, N, N // 2 + 1)
-    a = torch.zeros((nin,) + shape, dtype=cdt, device="cuda")
-    b = torch.zeros((max(nout, 1),) + shape, dtype=cdt, device="cuda")
-    C = a[0].numel() * a.element_size()
+    a = torch.zeros((nin, N, plane), dtype=cdt, device="cuda")
+    b = torch.zeros((max(nout, 1), N, plane), dtype=cdt, device="cuda")
+    C = N * N * (N // 2 + 1) * esz  # algorithmic bytes of one field (padding excluded)
     s = torch.cuda.current_stream().cuda_stream
     out = {}
     for tile in tiles:
@@ -401,8 +408,8 @@ def stream_ceiling(N=256, precision="fp64", mixes=((15, 6),), tiles=(0, 1, 2), x
             gbs = (r + w) * C / (ms * 1e-3) / 1e9
             out[f"tile{tile}_{r}r{w}w"] = {"ms": round(ms, 4), "GBps": round(gbs, 1)}
             if verbose:
-                print(f"[stream] N={N} {precision} tile {tile} {r} read + {w} written: {ms:.4f} ms {gbs:.0f} GB/s",
-                      flush=True)
+                print(f"[stream] N={N} {precision} pad {pad} tile {tile} {r} read + {w} written: {ms:.4f} ms "
+                      f"{gbs:.0f} GB/s", flush=True)
     del a, b
     torch.cuda.empty_cache()
     return out
@@ -487,6 +494,7 @@ def main(argv=None):
     st = sub.add_parser("stream")
     st.add_argument("--n", type=int, nargs="+", default=[256, 512])
     st.add_argument("--precision", nargs="+", default=["fp64", "fp32"])
+    st.add_argument("--pad", type=int, nargs="+", default=[None], help="plane padding, 128-byte lines")
     ra = sub.add_parser("ensemble-adjoint")
     ra.add_argument("--steps", type=int, default=200)
     ra.add_argument("--out", default=None)
@@ -502,7 +510,8 @@ def main(argv=None):
         # (9+9, 15+6, 12+6 fields), the x-inverse (3+6) and a plain 1+1
         for n in a.n:
             for prec in a.precision:
-                stream_ceiling(n, prec, mixes=((9, 9), (15, 6), (12, 6), (3, 6), (1, 1)))
+                for pad in a.pad:
+                    stream_ceiling(n, prec, mixes=((9, 9), (15, 6), (12, 6), (3, 6), (1, 1)), pad=pad)
         return
     if a.cmd == "fft":
         fft_throughput(tuple(a.n), reps=a.reps)
