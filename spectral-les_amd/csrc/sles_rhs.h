@@ -191,11 +191,16 @@ struct XRows {
 };
 
 // A y-pass tile that lies in the Nyquist part of the workspace arrangement
-// (whole tiles only: TK divides N N/2).  Its lines -- the Nyquist column of
-// TK planes -- are contiguous, so its lanes run along them.
+// (whole tiles only: TK divides N N/2) on a grid of at most 128^3.  Its
+// lines -- the Nyquist column of TK planes -- are contiguous, so its lanes
+// run along them: on the small grids, whose passes last as long as their
+// slowest tile, loads scattered over TK planes made that tile the slowest
+// (64^3 FP64 y-inverse 0.038 against 0.024 ms).  On the larger grids those
+// few tiles are hidden among thousands, and the column-major lanes keep their
+// LDS exchanges free of bank conflicts.
 template <int N, int TK>
 __device__ __forceinline__ bool y_nyquist_tile(int xcd) {
-  return (Split<N>::MAIN % TK == 0) && tile_id(xcd) * TK >= Split<N>::MAIN;
+  return N <= 128 && (Split<N>::MAIN % TK == 0) && tile_id(xcd) * TK >= Split<N>::MAIN;
 }
 
 // Rows of column q of a y-pass tile (q < N (N/2+1)) in the workspace
