@@ -438,17 +438,18 @@ __device__ __forceinline__ unsigned z_nyquist(long r, long pw) {
   return (unsigned)((r / N) * pw + Split<N>::MAIN + r % N);
 }
 
-// Nyquist bins of fields f < NF (field stride C) for the block's LPB lines,
-// destined for nq[f * LPB + line] (lines past the last row read row 0).
+// Nyquist bins of fields f < NF (field stride C) for the block's LPB lines
+// (rows row_base ...), destined for nq[f * LPB + line] (lines past the last
+// row read row 0).
 // When the block has at least NF LPB lanes, lane i issues the load of value
 // i (every lane loads, branch-free; lanes past NF LPB read value 0 again) and
 // nyquist_stage stores it later, so the load's latency overlaps whatever is
 // issued in between; otherwise nyquist_issue loads them all into nq itself.
 template <typename T, int N, int LPB, int NF, int THREADS>
-__device__ __forceinline__ cpx<T> nyquist_issue(cpx<T>* nq, const cpx<T>* W, long C, long pw) {
+__device__ __forceinline__ cpx<T> nyquist_issue(cpx<T>* nq, const cpx<T>* W, long C, long pw, long row_base) {
   auto value = [&](int i) {
     const int f = i / LPB, l = i % LPB;
-    long r = (long)blockIdx.x * LPB + l;
+    long r = row_base + l;
     if (r >= (long)N * N) r = 0;
     return ld32(W + f * C, z_nyquist<N>(r, pw));
   };
@@ -468,11 +469,11 @@ __device__ __forceinline__ void nyquist_stage(cpx<T>* nq, cpx<T> v) {
 }
 // nq[f * LPB + line] -> the Nyquist bins of fields f < nf (valid lines)
 template <typename T, int N, int LPB, bool WAVE>
-__device__ __forceinline__ void nyquist_store(const cpx<T>* nq, cpx<T>* W, long C, int nf, long pw) {
+__device__ __forceinline__ void nyquist_store(const cpx<T>* nq, cpx<T>* W, long C, int nf, long pw, long row_base) {
   line_sync<WAVE>();
   for (int i = threadIdx.x; i < nf * LPB; i += blockDim.x) {
     const int f = i / LPB, l = i % LPB;
-    const long r = (long)blockIdx.x * LPB + l;
+    const long r = row_base + l;
     if (r < (long)N * N) st32(W + f * C, z_nyquist<N>(r, pw), nq[i]);
   }
 }
@@ -573,7 +574,7 @@ __device__ __forceinline__ void block_max_store(double v, double* out) {
 
 template <typename T, int N, class PL, int MODEL>
 __global__ void __launch_bounds__(ZLines<PL>::THREADS)
-    k_zles(cpx<T>* W, long pw, T scale, T coeff, double* umax_part, const cpx<T>* __restrict__ tw, Members mb) {
+    k_zles(cpx<T>* W, long pw, T scale, T coeff, double* umax_part, const cpx<T>* __restrict__ tw, int xcd, Members mb) {
   W = member_ptr(W, mb.work);
   if (umax_part) umax_part = member_ptr(umax_part, mb.work);
   if (mb.coeff) coeff = (T)mb.coeff[blockIdx.y];
@@ -587,7 +588,10 @@ __global__ void __launch_bounds__(ZLines<PL>::THREADS)
   constexpr bool NQ = sizeof(T) == 4;  // Nyquist bins staged in LDS (z_row0)
   __shared__ cpx<T> nq[!NQ ? 1 : 9 * LPB > TH ? 9 * LPB : TH];  // Nyquist bins of the block's lines: 9 inputs, then 8 outputs
   const int line = threadIdx.x / NT, t = threadIdx.x % NT;
-  const long row = (long)blockIdx.x * LPB + line;
+  // lines in XCD-aware block order: consecutive rows y of a plane -- whose
+  // Nyquist bins share cache lines -- run in the same XCD's L2
+  const long row_base = tile_id(xcd) * LPB;
+  const long row = row_base + line;
   const bool valid = row < (long)N * N;
   const unsigned row0 = z_row0<N>(valid ? row : 0, pw);  // (x, y) line: plane x = row / N, row y = row % N
   const long C = N * pw;
@@ -614,7 +618,7 @@ __global__ void __launch_bounds__(ZLines<PL>::THREADS)
   // follow the first pair's, so their latencies overlap
   PairRaw<T, P, NQ> pr;
   cpx<T> nqv{};
-  if constexpr (NQ) nqv = nyquist_issue<T, N, LPB, 9, TH>(nq, W, C, pw);
+  if constexpr (NQ) nqv = nyquist_issue<T, N, LPB, 9, TH>(nq, W, C, pw, row_base);
   pair_issue<T, N, P, NT>(pr, w[0], w[1], row0, nyq_addr, t);
   if constexpr (NQ) nyquist_stage<T, LPB, 9, TH, WV>(nq, nqv);
 #define SLES_INV(FA, FB, DA, DB, NA, NB, RA, RB)                                     \
@@ -731,12 +735,12 @@ __global__ void __launch_bounds__(ZLines<PL>::THREADS)
     if (valid) st32(w[3], row0 + t + m * NT, A[m]);
   put_nyq(3, A[P / 2]);
 #undef SLES_FWD
-  if constexpr (NQ) nyquist_store<T, N, LPB, WV>(nq, W, C, 8, pw);
+  if constexpr (NQ) nyquist_store<T, N, LPB, WV>(nq, W, C, 8, pw, row_base);
 }
 
 template <typename T, int N, class PL>
 __global__ void __launch_bounds__(ZLines<PL>::THREADS)
-    k_zns(cpx<T>* W, long pw, T scale, double* umax_part, const cpx<T>* __restrict__ tw, Members mb) {
+    k_zns(cpx<T>* W, long pw, T scale, double* umax_part, const cpx<T>* __restrict__ tw, int xcd, Members mb) {
   W = member_ptr(W, mb.work);
   if (umax_part) umax_part = member_ptr(umax_part, mb.work);
   constexpr int P = PL::P, NT = PL::NT, NH = N / 2 + 1, KH = P / 2 + 1;
@@ -749,7 +753,10 @@ __global__ void __launch_bounds__(ZLines<PL>::THREADS)
   constexpr bool NQ = sizeof(T) == 4;  // Nyquist bins staged in LDS (z_row0)
   __shared__ cpx<T> nq[!NQ ? 1 : 6 * LPB > TH ? 6 * LPB : TH];  // Nyquist bins of the block's lines: 6 inputs, then 3 outputs
   const int line = threadIdx.x / NT, t = threadIdx.x % NT;
-  const long row = (long)blockIdx.x * LPB + line;
+  // lines in XCD-aware block order: consecutive rows y of a plane -- whose
+  // Nyquist bins share cache lines -- run in the same XCD's L2
+  const long row_base = tile_id(xcd) * LPB;
+  const long row = row_base + line;
   const bool valid = row < (long)N * N;
   const unsigned row0 = z_row0<N>(valid ? row : 0, pw);  // (x, y) line: plane x = row / N, row y = row % N
   const long C = N * pw;
@@ -774,7 +781,7 @@ __global__ void __launch_bounds__(ZLines<PL>::THREADS)
   // follow the first pair's, so their latencies overlap
   PairRaw<T, P, NQ> pr;
   cpx<T> nqv{};
-  if constexpr (NQ) nqv = nyquist_issue<T, N, LPB, 6, TH>(nq, W, C, pw);
+  if constexpr (NQ) nqv = nyquist_issue<T, N, LPB, 6, TH>(nq, W, C, pw, row_base);
   pair_issue<T, N, P, NT>(pr, w[0], w[1], row0, nyq_addr, t);
   if constexpr (NQ) nyquist_stage<T, LPB, 6, TH, WV>(nq, nqv);
 #define SLES_INV(FA, NA, RA, RB)                                                            \
@@ -820,7 +827,7 @@ __global__ void __launch_bounds__(ZLines<PL>::THREADS)
   for (int m = 0; m < P / 2; ++m)
     if (valid) st32(w[2], row0 + t + m * NT, A[m]);
   put_nyq(2, A[P / 2]);
-  if constexpr (NQ) nyquist_store<T, N, LPB, WV>(nq, W, C, 3, pw);
+  if constexpr (NQ) nyquist_store<T, N, LPB, WV>(nq, W, C, 3, pw, row_base);
 }
 
 // ---------------------------------------------------------------------------
@@ -1342,9 +1349,11 @@ struct Fused {
       { ProfScope ps_(PK_YINV, s); hipLaunchKernelGGL((k_yinv<T, N, 1, PS>), gs, bs, 0, s, W, pw, tw, xcd, mb); }
       ProfScope ps_(PK_Z, s);
       if (les == SLES_LES_SMAGORINSKY)
-        hipLaunchKernelGGL((k_zles<T, N, PZ, SLES_LES_SMAGORINSKY>), gz, bz, 0, s, W, pw, scale, coeff, part, tw, mb);
+        hipLaunchKernelGGL((k_zles<T, N, PZ, SLES_LES_SMAGORINSKY>), gz, bz, 0, s, W, pw, scale, coeff, part, tw, xcd,
+                           mb);
       else
-        hipLaunchKernelGGL((k_zles<T, N, PZ, SLES_LES_VREMAN>), gz, bz, 0, s, W, pw, scale, coeff, part, tw, mb);
+        hipLaunchKernelGGL((k_zles<T, N, PZ, SLES_LES_VREMAN>), gz, bz, 0, s, W, pw, scale, coeff, part, tw, xcd,
+                           mb);
     } else {
       if (!skip_xinv) {
         ProfScope ps_(PK_XINV, s);
@@ -1352,7 +1361,7 @@ struct Fused {
       }
       { ProfScope ps_(PK_YINV, s); hipLaunchKernelGGL((k_yinv<T, N, 0, PS>), gs, bs, 0, s, W, pw, tw, xcd, mb); }
       ProfScope ps_(PK_Z, s);
-      hipLaunchKernelGGL((k_zns<T, N, PZ>), gz, bz, 0, s, W, pw, scale, part, tw, mb);
+      hipLaunchKernelGGL((k_zns<T, N, PZ>), gz, bz, 0, s, W, pw, scale, part, tw, xcd, mb);
     }
     return check_launch("fused stage (inverse)") ? -1 : 0;
   }
