@@ -77,20 +77,24 @@ def kernel_traffic_fields(les):
     }
 
 
-def xfwd_mean_fields(les, fused=False):
-    """Mean (read + write) complex fields of the x-forward/update pass over the
-    4 RK4 stages: stage 0 reads W + U_hat and writes U_hat0, U_hat1, U_hat;
+def xfwd_mean_fields(les, fused=False, steps=3):
+    """Mean (read + write) complex fields per x-forward/update launch over one
+    driver call of `steps` RK4 steps (sles_step_drive, U and curl written at
+    its end).  Stage 0 reads W + U_hat and writes U_hat0, U_hat1, U_hat;
     stages 1-2 read W + U_hat, U_hat0, U_hat1 and write U_hat1, U_hat;
     stage 3 reads W + U_hat, U_hat1 and writes U_hat1, U_hat.
-    fused (FP32 form): stages 0-2 also write the next stage's x-inverse
-    output (6 fields with LES, 5 without), stage 3 the physical-output
-    pass's (5 fields); those passes then read nothing more."""
+    fused (the plan's fuse_x): every launch also writes the output of the
+    x-inverse that follows it (6 fields with LES, 5 without) -- the next
+    stage's, or at stage 3 the next step's -- except the call's last stage 3,
+    which writes the physical-output pass's (5 fields); those x-inverse
+    passes then read nothing more."""
     w = 6 if les else 3
     per_stage = [w + 3 + 9, w + 9 + 6, w + 9 + 6, w + 6 + 6]
+    total = steps * sum(per_stage)
     if fused:
         xi = 6 if les else 5
-        per_stage = [f + xi for f in per_stage[:3]] + [per_stage[3] + 5]
-    return sum(per_stage) / 4.0
+        total += (4 * steps - 1) * xi + 5
+    return total / (4.0 * steps)
 
 
 def step_bytes_minimal(N, precision, les):
@@ -183,7 +187,7 @@ def run_gpu(N, precision, steps, warmup, profile_steps=3, chunk=10):
         _lib.call("sles_profile_enable", 0)
         kern = {KINDS[i]: (ms_k[i], cnt_k[i]) for i in range(7)}
     out = dict(ms_total=ms, ms_total_step_each=ms_each, steps=steps, dt_last=float(st[1]),
-               umax_last=float(st[0]), kernels=kern, ok=ok, tuning=d.plan.get_tuning())
+               umax_last=float(st[0]), kernels=kern, ok=ok, tuning=d.plan.get_tuning(), profile_steps=profile_steps)
     del d
     torch.cuda.empty_cache()
     return out
@@ -338,20 +342,25 @@ def tgv_cpu(N=64, steps=20):
             "sample": f"{steps} RK4 steps of the 64^3 Taylor-Green case, numpy pocketfft, {t:.2f} s"}
 
 
-def x_forward_pattern_ceiling(N, precision, pad):
+def x_forward_pattern_ceiling(N, precision, pad, fused=True):
     """The x-forward + update pass's own access pattern with no arithmetic
     (sles_stream_probe, the radix-4 x-forward tile, XCD order, the step's
-    padded plane layout): the mean over the 4 RK4 stages' traffic mixes
-    (9+9, 15+6, 15+6, 12+6 fields read + written) of the algorithmic GB/s --
-    the bandwidth this tile shape and traffic reach on this device, beside
-    the 8 TB/s peak."""
+    padded plane layout): the RK4 stages' traffic mixes -- 9+9, 15+6, 15+6,
+    12+6 fields read + written, and 6 more written by each launch that also
+    runs the next x-inverse (fused) -- in algorithmic GB/s: the bandwidth
+    this tile shape and traffic reach on this device, beside the 8 TB/s
+    peak."""
     from sles.studies import stream_ceiling
 
-    r = stream_ceiling(N, precision, mixes=((9, 9), (15, 6), (12, 6)), tiles=(0,), verbose=False, pad=pad)
+    xi = 6 if fused else 0
+    mixes = ((9, 9 + xi), (15, 6 + xi), (12, 6 + xi))
+    r = stream_ceiling(N, precision, mixes=mixes, tiles=(0,), verbose=False, pad=pad)
     R, C = field_bytes(N, precision)
-    ms = (r["tile0_9r9w"]["ms"] + 2 * r["tile0_15r6w"]["ms"] + r["tile0_12r6w"]["ms"]) / 4
-    nbytes = xfwd_mean_fields(True) * C
+    per_stage = (mixes[0], mixes[1], mixes[1], mixes[2])  # stages 0-3
+    ms = sum(r[f"tile0_{a}r{b}w"]["ms"] for a, b in per_stage) / 4
+    nbytes = sum(a + b for a, b in per_stage) / 4 * C
     return {"GBps": round(nbytes / (ms * 1e-3) / 1e9, 1), "avg_launch_ms": round(ms, 4), "pad_lines": pad,
+            "fields_per_launch": sum(a + b for a, b in per_stage) / 4,
             "kernel": "sles_stream_probe tile 0 (loads and stores of the x-forward tile, no arithmetic)"}
 
 
@@ -362,10 +371,11 @@ def roofline_of(res, N, precision, les=True):
     best = max(fields, key=lambda k: kern[k][0])
     tot_ms, cnt = kern[best]
     avg_s = tot_ms / cnt / 1e3
-    # the FP32 form fuses the next RK4 stage's x-inverse into the x-forward
-    fused = precision == "fp32"
+    # x-forward launches that also run the next x-inverse (plan knob fuse_x)
+    fused = bool(res["tuning"].get("fuse_x", 0))
+    steps = res["profile_steps"]
     if best == "x_forward_update":
-        nbytes = xfwd_mean_fields(les, fused) * C
+        nbytes = xfwd_mean_fields(les, fused, steps) * C
     else:
         nbytes = sum(fields[best]) * C
     achieved = nbytes / avg_s / 1e9
@@ -375,7 +385,7 @@ def roofline_of(res, N, precision, les=True):
             continue
         entry = {"ms_per_launch": round(t / c, 4), "launches": c}
         if k in fields:
-            b = xfwd_mean_fields(les, fused) * C if k == "x_forward_update" else sum(fields[k]) * C
+            b = xfwd_mean_fields(les, fused, steps) * C if k == "x_forward_update" else sum(fields[k]) * C
             entry["algorithmic_GBps"] = round(b / (t / c / 1e3) / 1e9, 1)
         per_kernel[k] = entry
     return {
@@ -505,7 +515,8 @@ def main(argv=None):
     ms_step = r64["ms_total"] / a.steps
     value = 1e3 / ms_step
     roof, per_kernel = roofline_of(r64, N, "fp64")
-    roof["pattern_ceiling"] = x_forward_pattern_ceiling(N, "fp64", r64["tuning"]["pad"])
+    roof["pattern_ceiling"] = x_forward_pattern_ceiling(N, "fp64", r64["tuning"]["pad"],
+                                                        bool(r64["tuning"].get("fuse_x", 0)))
     step_bytes = step_bytes_minimal(N, "fp64", True)
     log(f"[bench] fp64 {value:.2f} steps/s ({ms_step:.3f} ms/step); dominant {roof['kernel']} "
         f"{roof['achieved']} GB/s")

@@ -54,9 +54,8 @@ int sles_plan_info(sles_plan plan, int* n, int* nh, int* precision);
  *   SLES_TUNE_XCD             XCD-aware tile order in the strided passes (fused
  *                             step and the generic transforms)
  *   SLES_TUNE_FUSE_X          RK4 x-forward pass also runs the next stage's
- *                             x-inverse (1, default) or not (0); the fused
- *                             kernels exist in FP32 and for FP64 N <= 64
- *                             (ignored otherwise)
+ *                             x-inverse (1, default) or not (0), in both
+ *                             precisions at every N
  *   SLES_TUNE_FFT_WIDE        generic transforms (sles_fft_r2c / sles_fft_c2r):
  *                             bit 0 = x pass, bit 1 = y pass run tiles of
  *                             twice the columns (256- or 512-byte row runs, LDS tile
@@ -333,9 +332,11 @@ int sles_step_drive_batch(sles_plan plan, int scheme, const sles_step_params* pa
  * workspace is a static device array of SLES_STEP_SCRATCH_BYTES (64 MiB)
  * that every device running the library holds until the process ends; its
  * address is taken per device by sles_plan_create.  Calls whose workspace
- * (sles_step_work_bytes) exceeds it are rejected with an error: with LES up
- * to N = 96 in FP64 and N = 96 in FP32 (128 in FP32 needs 73 MiB), without
- * LES up to 96 (FP64) and 128 (FP32).  Every call on a device shares that
+ * -- the W fields and CFL partials of sles_step_work_bytes, without the
+ * drivers' copies of the RK state that it also counts when SLES_TUNE_PAD > 0
+ * -- exceeds it are rejected with an error: with LES up to N = 96 in FP64
+ * and N = 96 in FP32 (128 in FP32 needs 73 MiB), without LES up to 96
+ * (FP64) and 128 (FP32).  Every call on a device shares that
  * device's array, so calls that overlap on one device (two streams, two
  * threads) race; each call writes the array before it reads it, so nothing
  * carries over between calls.  Same arguments, results and errors as

@@ -69,10 +69,14 @@ struct StepArgs {
 };
 
 // sles_step_drive: the driver-loop state (device doubles, DRV_* layout) and
-// the CFL of the first step.
+// the CFL of the first step.  Between two RK4 steps of one driver call, the
+// last stage's x-forward can run the next step's first x-inverse (the plan's
+// fuse_x knob): xinv_out asks a step to do so, xinv_in tells a step that its
+// workspace already holds that x-inverse (the previous step was asked).
 struct DriveArgs {
   double* state;
   double cfl_first;
+  bool xinv_in = false, xinv_out = false;
 };
 
 // Batched launches (sles_step_drive_batch): blockIdx.y is the member, whose
@@ -1032,9 +1036,10 @@ using XShape = StridedShape<T, N, PL, (sizeof(T) == 4 && PL::P == 4) ? 1024 : 51
 // same columns: no other block touches them), writing W as k_xinv would, so
 // the next stage skips k_xinv and the re-read of U_hat.  FP32 256^3: 0.479 ms
 // per fused launch against 0.372 + 0.146 ms separately (step +2 %).  In FP64
-// the fused inverse runs on the x-forward's radix-4 plan at 2 waves per SIMD
-// and was slower than the separate radix-8 k_xinv (1.071 against 1.038 ms),
-// so FP64 keeps the two kernels except on launch-bound grids (Fused::FUSE_X).
+// the fused inverse runs on the x-forward's radix-4 plan at 2 waves per SIMD:
+// over planes a multiple of 4 KiB apart it was slower than the separate
+// radix-8 k_xinv (256^3: 1.071 against 1.038 ms); over padded planes it saves
+// 0.1 ms of the 0.8 ms that the two kernels take.
 // FLES: layout of the fused x-inverse output (LES stage layout, or 0: the
 // layout without LES, which is also the physical-output pass's: the last RK4
 // stage runs that pass's x-inverse on the new U_hat).
@@ -1324,11 +1329,11 @@ struct Fused {
   using SS = WideShape<T, N, PS>;  // x/y inverse and y forward
   using SX = XShape<T, N, PX>;  // x forward + stage update
   static constexpr int ZB = (int)(((long)N * N + ZLines<PZ>::LPB - 1) / ZLines<PZ>::LPB);
-  // RK4 x-forward can run the next stage's x-inverse (see k_xfwd; plan knob
-  // tune_fuse_x): FP32 form, and FP64 grids up to 64^3 (launch-bound: one
-  // launch fewer per stage; 64^3 Taylor-Green 0.323 -> 0.312 ms per step; at
-  // 128^3 FP64 the fused form was 2 % slower)
-  static constexpr bool FUSE_X = sizeof(T) == 4 || N <= 64;
+  // RK4 x-forward can run the next stage's x-inverse (see k_xfwd), in both
+  // forms and at every size; the plan knob tune_fuse_x chooses per grid
+  // (64^3 Taylor-Green 0.323 -> 0.312 ms per step; 256^3 FP64 Smagorinsky
+  // through the driver loop over the padded workspace 8.49 -> 8.30 ms)
+  static constexpr bool FUSE_X = true;
 
   // skip_xinv: W0-5 already hold the x-inverse output (fused into the previous x-forward)
   // nm members (grid rows) at the strides of mb; ps: kx-plane stride of the RK-state arrays
@@ -1471,7 +1476,11 @@ int step_impl(const sles_plan_s* p, int scheme, const sles_step_params* prm, voi
     a.cb = rk4b[st];
     double* pp = (ddt && st == 0) ? part : nullptr;
     const bool fuse = F::FUSE_X && p->tune_fuse_x && scheme == SLES_SCHEME_RK4;
-    const bool fused_in = fuse && st > 0;  // x-inverse ran in the previous x-forward
+    // the x-inverse ran in the previous x-forward (of this step, or of the
+    // driver's previous step)
+    const bool fused_in = fuse && (st > 0 || (drive && drive->xinv_in));
+    // stage 3 runs the driver's next step's x-inverse, or the physical pass's
+    const bool next3 = fuse && drive && drive->xinv_out;
     if (F::stage(p, les, coeff, uh, ps, ssplit, W, pp, s, fused_in, mb, nm)) return -1;
     if (pp) {
       // two levels: up to 64 chunks of the per-block maxima, then one block
@@ -1490,12 +1499,13 @@ int step_impl(const sles_plan_s* p, int scheme, const sles_step_params* prm, voi
                            scalars);
       if (check_launch("cfl")) return -1;
     }
-    if (F::finish(p, les, uh, uh0, uh1, aux, ps, ssplit, W, a, s, fuse && st < 3,
-                  fuse && st == 3 && (u_out || curl_out), mb, nm))
+    if (F::finish(p, les, uh, uh0, uh1, aux, ps, ssplit, W, a, s, fuse && (st < 3 || next3),
+                  fuse && st == 3 && !next3 && (u_out || curl_out), mb, nm))
       return -1;
   }
   // the last fused RK4 x-forward already ran the physical pass's x-inverse
-  const bool phys_fused = F::FUSE_X && p->tune_fuse_x && scheme == SLES_SCHEME_RK4;
+  // (unless it ran the next step's)
+  const bool phys_fused = F::FUSE_X && p->tune_fuse_x && scheme == SLES_SCHEME_RK4 && !(drive && drive->xinv_out);
   if (u_out || curl_out) return F::physical(p, uh, ps, ssplit, W, u_out, curl_out, s, phys_fused);
   return 0;
 }

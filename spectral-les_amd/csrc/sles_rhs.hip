@@ -228,7 +228,10 @@ int sles_step(sles_plan p, int scheme, const sles_step_params* prm, void* u_hat,
 int sles_step_noalloc(sles_plan p, int scheme, const sles_step_params* prm, void* u_hat, void* u_hat0,
                       void* u_hat1, void* aux, void* u_out, void* curl_out, double* scalars, void* stream) {
   if (!p || !prm) return fail("sles_step_noalloc: null argument");
-  const size_t need = sles_step_work_bytes(p, prm->les_model);
+  if (prm->les_model < 0 || prm->les_model > 2) return fail("sles_step_noalloc: les_model must be 0, 1 or 2");
+  // a single step uses the W fields and the CFL partials only (the drivers'
+  // copies of the RK state that sles_step_work_bytes also counts are unused)
+  const size_t need = sles::work_fields_bytes(p, prm->les_model);
   if (need > (size_t)SLES_STEP_SCRATCH_BYTES)
     return fail("sles_step_noalloc: the step needs " + std::to_string(need) + " bytes of workspace, above the " +
                 std::to_string((size_t)SLES_STEP_SCRATCH_BYTES) + "-byte device scratch (N = " +
@@ -258,7 +261,7 @@ int sles_step_drive(sles_plan p, int scheme, const sles_step_params* prm, double
   if (!sles_supported_n(p->n)) return fail("sles_step_drive: unsupported grid size");
   hipStream_t s = (hipStream_t)stream;
   if (nsteps == 0) return 0;
-  const DriveArgs drv{state, cfl_first};
+  DriveArgs drv{state, cfl_first};
   const long sbytes = 3 * ncplx(p) * cplx_bytes(p);
   DriveState ds{};
   if (drive_enter(p, scheme, prm->les_model, u_hat, u_hat0, u_hat1, aux, 1, sbytes, state, 0, work, 0, s, &ds))
@@ -270,6 +273,8 @@ int sles_step_drive(sles_plan p, int scheme, const sles_step_params* prm, double
     const bool last = i + 1 == nsteps;
     void* uo = last ? u_out : nullptr;
     void* co = last ? curl_out : nullptr;
+    drv.xinv_in = i > 0;  // read only where the step fuses (RK4, fuse_x)
+    drv.xinv_out = !last;
     const int rc =
         p->precision == SLES_FP64
             ? step_f64(p, scheme, prm, ds.uh, ds.uh0, ds.uh1, ds.aux, ds.ps, ds.split, uo, co, nullptr, work, s,
@@ -323,8 +328,10 @@ int sles_step_drive_batch(sles_plan p, int scheme, const sles_step_params* prm, 
   b.mb.work = wstride;
   b.mb.drive = dstride;
   b.mb.coeff = prm->les_model ? sgs_coeff : nullptr;
-  const DriveArgs drv{state, cfl_first};
+  DriveArgs drv{state, cfl_first};
   for (int i = 0; i < nsteps; ++i) {
+    drv.xinv_in = i > 0;  // read only where the step fuses (RK4, fuse_x)
+    drv.xinv_out = i + 1 < nsteps;
     const int rc = p->precision == SLES_FP64
                        ? step_f64(p, scheme, prm, ds.uh, ds.uh0, ds.uh1, ds.aux, ds.ps, ds.split, nullptr, nullptr,
                                   nullptr, work, s, nullptr, &drv, &b)

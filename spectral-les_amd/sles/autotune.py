@@ -64,16 +64,10 @@ def apply(plan, N, precision, les_model, path=TABLE):
     return entry["settings"]
 
 
-def has_fused_x(N, precision):
-    """The fused x-forward + next x-inverse kernels are built for FP32 and for
-    FP64 grids up to 64^3."""
-    return precision == "fp32" or N <= 64
-
-
 def candidates(N, precision=None):
+    """Every kernel variant of a grid size (the same set in both precisions)."""
     radix = (0, 1) if N in RADIX4_SIZES else (0,)
-    fuse = (0, 1) if precision is None or has_fused_x(N, precision) else (1,)
-    for s, x, z, c, f in itertools.product(radix, radix, radix, (0, 1), fuse):
+    for s, x, z, c, f in itertools.product(radix, radix, radix, (0, 1), (0, 1)):
         yield dict(strided_radix4=s, xfwd_radix4=x, z_radix4=z, xcd=c, fuse_x=f)
 
 

@@ -29,8 +29,7 @@ def _same(a, b, tag):
 def test_batched_matches_per_member(precision, les_model):
     """Member offsets of every kernel of the step (x/y passes, z kernel,
     x-forward + update, both CFL reduction levels) and the per-member SGS
-    coefficient: groups of 3, 2 and 2 members (96^3 takes the FP64 form's
-    unfused x passes, the smaller grids the fused ones)."""
+    coefficient: groups of 3, 2 and 2 members."""
     from sles.ensemble import Ensemble
 
     groups = ((16, 3), (24, 2), (96, 2))

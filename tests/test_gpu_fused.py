@@ -162,7 +162,7 @@ def test_taylor_green_64_rk4_trajectory():
     assert rel(g.U_hat, o.U_hat) < 1e-10
 
 
-@pytest.mark.parametrize("N", [32, 48])  # 48: 12-point plan and its 6-point alternative
+@pytest.mark.parametrize("N", [32, 48, 128])  # 48: 12-point plan and its 6-point alternative
 @pytest.mark.parametrize("les", [None, "Smagorinsky", "Vreman"])
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 def test_every_tuning_variant_matches_oracle(les, precision, N):
