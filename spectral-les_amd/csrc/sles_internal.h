@@ -90,18 +90,26 @@ struct ZShape {
   static constexpr int NH = N / 2 + 1;
 };
 
+constexpr long pow2_floor(long v) { return v < 2 ? 1 : 2 * pow2_floor(v / 2); }
+
 // Strided passes: TK columns per block, LDS tile N*TK complex <= 64 KiB.
 // Blocks of plans with >= 6 points per thread are capped at 512 threads (8
 // waves: up to 256 VGPRs per wave without spilling); radix-4 plans (4 points
 // per thread, about half the registers) may use 1024.  MAXT_ > 0 overrides
-// the cap for one kernel.
+// the cap for one kernel.  Small grids have few columns (N (N/2+1): 2112 at
+// 64^3), so tiles of the LDS width would leave most CUs idle (33 blocks at
+// 64^3 FP64); there a tile spans at most N (N/2+1) / 256 columns (one tile
+// per CU), but at least one wavefront's worth (64 / NT).
 template <typename T, int N, class PL = LinePlan<N>, int MAXT_ = 0>
 struct StridedShape {
   static constexpr int P = PL::P;
   static constexpr int NT = N / P;
   static constexpr int MAXT = MAXT_ > 0 ? MAXT_ : (P >= 6 ? 512 : 1024);
   static constexpr int TK0 = 65536 / (N * (int)sizeof(cpx<T>));
-  static constexpr int TK1 = TK0 > 64 ? 64 : TK0;
+  static constexpr int FILL0 = (int)pow2_floor((long)N * (N / 2 + 1) / 256);
+  static constexpr int FILL = FILL0 > 64 / NT ? FILL0 : (64 / NT > 0 ? 64 / NT : 1);
+  static constexpr int TKL = TK0 > 64 ? 64 : TK0;
+  static constexpr int TK1 = TKL < FILL ? TKL : FILL;
   static constexpr int TK = (TK1 * NT > MAXT) ? (MAXT / NT > 0 ? MAXT / NT : 1) : TK1;
   static constexpr int THREADS = TK * NT;
 };
