@@ -68,6 +68,13 @@ int sles_plan_info(sles_plan plan, int* n, int* nh, int* precision);
  *                             (0..SLES_TUNE_PAD_MAX); 0 steps the caller's
  *                             arrays in place.  sles_step_work_bytes depends
  *                             on it: query it after setting this key
+ *   SLES_TUNE_NARROW          fused step, N = 64, 96, 128: the strided and
+ *                             x-forward passes take tiles of N (N/2+1) / 1024
+ *                             columns (four per CU; 1) instead of / 256 (one
+ *                             per CU; 0, default) -- faster for one grid,
+ *                             slower for batched launches, which fill the CUs
+ *                             with their members; results are identical
+ *                             (ignored at other sizes)
  * The "radix-4" alternative plans: 4 points per thread (radix 4) for
  * N = 32, 64, 128, 256, 512; 6 points per thread (radix 3, then 2) for
  * N = 12, 24, 48, 96, 192, 384; ignored for N = 8, 16. */
@@ -78,7 +85,8 @@ enum {
   SLES_TUNE_XCD = 3,
   SLES_TUNE_FFT_WIDE = 4,
   SLES_TUNE_FUSE_X = 5,
-  SLES_TUNE_PAD = 6
+  SLES_TUNE_PAD = 6,
+  SLES_TUNE_NARROW = 7
 };
 #define SLES_TUNE_PAD_MAX 64
 int sles_plan_set_tuning(sles_plan plan, int key, int value);

@@ -442,6 +442,7 @@ int sles_plan_set_tuning(sles_plan p, int key, int value) {
         return fail("sles_plan_set_tuning: SLES_TUNE_PAD takes 0.." + std::to_string(SLES_TUNE_PAD_MAX));
       p->tune_pad = value;
       return 0;
+    case SLES_TUNE_NARROW: p->tune_narrow = value != 0; return 0;
     default: return fail("sles_plan_set_tuning: unknown key");
   }
 }
@@ -456,6 +457,7 @@ int sles_plan_get_tuning(sles_plan p, int key, int* value) {
     case SLES_TUNE_FFT_WIDE: *value = p->tune_fft_wide; return 0;
     case SLES_TUNE_FUSE_X: *value = p->tune_fuse_x; return 0;
     case SLES_TUNE_PAD: *value = p->tune_pad; return 0;
+    case SLES_TUNE_NARROW: *value = p->tune_narrow; return 0;
     default: return fail("sles_plan_get_tuning: unknown key");
   }
 }

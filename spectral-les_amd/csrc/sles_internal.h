@@ -24,6 +24,7 @@ struct sles_plan_s {
                          // (default set per size in sles_plan_create)
   int tune_pad = 0;      // fused-step workspace: kx planes padded by this many 128-byte lines
                          // (default set per size in sles_plan_create)
+  int tune_narrow = 0;   // fused step, N = 64..128: narrow small-grid tiles in the strided passes
 };
 
 namespace sles {
@@ -92,6 +93,20 @@ struct ZShape {
 
 constexpr long pow2_floor(long v) { return v < 2 ? 1 : 2 * pow2_floor(v / 2); }
 
+// Line plan PL whose strided passes take the narrow small-grid tiles (plan
+// knob SLES_TUNE_NARROW): four tiles per CU instead of one.  A tile's width
+// changes which block runs a column, not the arithmetic on it.
+template <class PL>
+struct Narrow : PL {};
+template <class PL>
+struct IsNarrow {
+  static constexpr bool value = false;
+};
+template <class PL>
+struct IsNarrow<Narrow<PL>> {
+  static constexpr bool value = true;
+};
+
 // Strided passes: TK columns per block, LDS tile N*TK complex <= 64 KiB.
 // Blocks of plans with >= 6 points per thread are capped at 512 threads (8
 // waves: up to 256 VGPRs per wave without spilling); radix-4 plans (4 points
@@ -99,14 +114,15 @@ constexpr long pow2_floor(long v) { return v < 2 ? 1 : 2 * pow2_floor(v / 2); }
 // the cap for one kernel.  Small grids have few columns (N (N/2+1): 2112 at
 // 64^3), so tiles of the LDS width would leave most CUs idle (33 blocks at
 // 64^3 FP64); there a tile spans at most N (N/2+1) / 256 columns (one tile
-// per CU), but at least one wavefront's worth (64 / NT).
+// per CU; Narrow plans: / 1024, four per CU), but at least one wavefront's
+// worth (64 / NT).
 template <typename T, int N, class PL = LinePlan<N>, int MAXT_ = 0>
 struct StridedShape {
   static constexpr int P = PL::P;
   static constexpr int NT = N / P;
   static constexpr int MAXT = MAXT_ > 0 ? MAXT_ : (P >= 6 ? 512 : 1024);
   static constexpr int TK0 = 65536 / (N * (int)sizeof(cpx<T>));
-  static constexpr int FILL0 = (int)pow2_floor((long)N * (N / 2 + 1) / 256);
+  static constexpr int FILL0 = (int)pow2_floor((long)N * (N / 2 + 1) / (IsNarrow<PL>::value ? 1024 : 256));
   static constexpr int FILL = FILL0 > 64 / NT ? FILL0 : (64 / NT > 0 ? 64 / NT : 1);
   static constexpr int TKL = TK0 > 64 ? 64 : TK0;
   static constexpr int TK1 = TKL < FILL ? TKL : FILL;

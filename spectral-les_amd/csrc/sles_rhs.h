@@ -1520,6 +1520,13 @@ struct HasP4 {
                                  N == 48 || N == 96 || N == 192 || N == 384);
 };
 
+// Sizes whose strided passes have narrow small-grid tiles to choose
+// (tune_narrow; the tile widths differ from 64^3 to 128^3 only).
+template <int N>
+struct HasNarrow {
+  static constexpr bool value = N == 64 || N == 96 || N == 128;
+};
+
 template <typename T, int N>
 int step_t(const sles_plan_s* p, int scheme, const sles_step_params* prm, void* uh, void* uh0, void* uh1, void* aux,
            long ps, int ssplit, void* u_out, void* curl_out, double* scalars, void* work, hipStream_t s,
@@ -1532,6 +1539,23 @@ int step_t(const sles_plan_s* p, int scheme, const sles_step_params* prm, void* 
   case K:                     \
     return step_impl<T, N, A, B, Cz>(p, scheme, prm, uh, uh0, uh1, aux, ps, ssplit, u_out, curl_out, scalars, \
                                      work, s, stages, drive, batch);
+    if constexpr (HasNarrow<N>::value) {
+      if (p->tune_narrow) {
+        // the strided and x-forward passes in narrow tiles (the z pass has no tiles)
+        using ND = Narrow<D>;
+        using NQ = Narrow<Q>;
+        switch (sel) {
+          SLES_SEL(0, ND, ND, D)
+          SLES_SEL(1, NQ, ND, D)
+          SLES_SEL(2, ND, NQ, D)
+          SLES_SEL(3, NQ, NQ, D)
+          SLES_SEL(4, ND, ND, Q)
+          SLES_SEL(5, NQ, ND, Q)
+          SLES_SEL(6, ND, NQ, Q)
+          SLES_SEL(7, NQ, NQ, Q)
+        }
+      }
+    }
     switch (sel) {
       SLES_SEL(0, D, D, D)
       SLES_SEL(1, Q, D, D)

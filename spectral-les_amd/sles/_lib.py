@@ -20,7 +20,8 @@ SLES_FP32 = 1
 SLES_LES_NONE = 0
 SLES_LES_SMAGORINSKY = 1
 SLES_LES_VREMAN = 2
-TUNE_KEYS = {"strided_radix4": 0, "xfwd_radix4": 1, "z_radix4": 2, "xcd": 3, "fft_wide": 4, "fuse_x": 5, "pad": 6}
+TUNE_KEYS = {"strided_radix4": 0, "xfwd_radix4": 1, "z_radix4": 2, "xcd": 3, "fft_wide": 4, "fuse_x": 5, "pad": 6,
+             "narrow": 7}
 
 _vp = ctypes.c_void_p
 _int = ctypes.c_int

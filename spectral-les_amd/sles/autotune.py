@@ -8,7 +8,9 @@ plan's threads per line and the 64 KiB LDS tile -- the XCD-aware tile order
 and the fusion of the x-forward with the next x-inverse) on a few RK4 steps
 of a synthetic field through the device driver loop, and keeps the fastest;
 then, with those variants, it times the padding of the workspace's kx planes
-(``pad``, in 128-byte lines, N >= 64).  For N = 256 and 512 it also
+(``pad``, in 128-byte lines, N >= 64) and, for N = 64, 96 and 128, the tile
+width of the strided passes (``narrow``: one or four tiles per CU).  For
+N = 256 and 512 it also
 times the four tile widths of the generic transforms' strided passes
 (``fft_wide``: twice the columns per tile in the x and/or y pass) on a
 batched forward + backward transform.  Results persist in a JSON table
@@ -35,11 +37,12 @@ import torch
 from . import _lib
 
 TABLE = os.environ.get("SLES_TUNING_TABLE", os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning.json"))
-KNOBS = ("strided_radix4", "xfwd_radix4", "z_radix4", "xcd", "fuse_x", "fft_wide", "pad")
+KNOBS = ("strided_radix4", "xfwd_radix4", "z_radix4", "xcd", "fuse_x", "fft_wide", "pad", "narrow")
 WIDE_SIZES = (256, 512)  # sizes with wide generic-transform tiles
 DEFAULTS = dict(strided_radix4=0, xfwd_radix4=0, z_radix4=0, xcd=1, fuse_x=1)  # a new plan's settings
 PADS = (0, 5, 17, 33)  # workspace plane padding tried, in 128-byte lines
 PAD_MIN_N = 64  # smaller grids keep the compact layout (launch bound)
+NARROW_SIZES = (64, 96, 128)  # sizes with narrow small-grid tiles to choose
 RADIX4_SIZES = (12, 24, 32, 48, 64, 96, 128, 192, 256, 384, 512)  # sizes with an alternative line plan
 
 
@@ -156,6 +159,14 @@ def tune(N, precision="fp64", les="smagorinsky", steps=3, path=TABLE, verbose=Tr
         best_ms, best["pad"] = min(pads)
     else:
         best["pad"] = 0
+    # the strided passes' tile width on small grids (one or four tiles per
+    # CU), with the best variants and padding
+    if N in NARROW_SIZES:
+        narrow = [(time_variant(d, dict(best, narrow=w), steps=steps), w) for w in (0, 1)]
+        if verbose:
+            for ms, w in narrow:
+                print(f"[autotune] N={N} {precision} {les}: narrow={w} -> {ms:.3f} ms/step", flush=True)
+        best_ms, best["narrow"] = min(narrow)
     if N in WIDE_SIZES:
         d.plan.set_tuning(**best)
         wide = [(time_fft_wide(d, w), w) for w in range(4)]

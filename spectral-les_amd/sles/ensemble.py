@@ -69,6 +69,9 @@ class Group:
         self.plan = d0.plan
         self.cfl_first = float(cfl_first)
         M = len(members)
+        if M > 1:
+            # the batch's grid rows already fill the CUs: one tile per CU
+            self.plan.set_tuning(narrow=0)
         dev = d0.device
         shape = (M,) + tuple(d0.U_hat.shape)
         self.U_hat = torch.empty(shape, dtype=d0.cdtype, device=dev)

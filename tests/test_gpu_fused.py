@@ -162,7 +162,7 @@ def test_taylor_green_64_rk4_trajectory():
     assert rel(g.U_hat, o.U_hat) < 1e-10
 
 
-@pytest.mark.parametrize("N", [32, 48, 128])  # 48: 12-point plan and its 6-point alternative
+@pytest.mark.parametrize("N", [32, 48, 96, 128])  # 48, 96: 12-point plans and their 6-point alternatives
 @pytest.mark.parametrize("les", [None, "Smagorinsky", "Vreman"])
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 def test_every_tuning_variant_matches_oracle(les, precision, N):
@@ -177,10 +177,19 @@ def test_every_tuning_variant_matches_oracle(les, precision, N):
     for st in autotune.candidates(N):
         g.U_hat.copy_(start)
         g.U_hat1.copy_(start)
-        g.plan.set_tuning(**st)
+        g.plan.set_tuning(narrow=0, **st)
         assert {k: v for k, v in g.plan.get_tuning().items() if k in st} == st
         fused.rk4_step(g)
         assert rel(g.U_hat, o.U_hat) < tol, st
+        if N in autotune.NARROW_SIZES:
+            # narrow small-grid tiles: other blocks, the same arithmetic
+            wide = g.U_hat.clone()
+            g.U_hat.copy_(start)
+            g.U_hat1.copy_(start)
+            g.plan.set_tuning(narrow=1)
+            fused.rk4_step(g)
+            g.plan.set_tuning(narrow=0)
+            assert torch.equal(g.U_hat, wide), ("narrow", st)
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])

@@ -221,3 +221,26 @@ def test_batched_drive_padded_equals_compact(precision):
         assert torch.equal(a.states(), b.states())
     assert bool((b.states()[:, 4] == 3).all())
     assert np.isfinite(b.members[-1].U_hat.abs().max().item())
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("les", [None, "Smagorinsky"])
+@pytest.mark.parametrize("N", [64, 96, 128])
+def test_narrow_tiles_drive_bitwise(N, les, precision):
+    """SLES_TUNE_NARROW (four strided tiles per CU instead of one) moves
+    columns between blocks but not the arithmetic on them: the driver loop
+    over several calls gives the same states bit for bit."""
+    from sles import fused
+
+    a = _domain(N, precision, les, "RK4", 5)
+    b = _domain(N, precision, les, "RK4", 5)
+    b.plan.set_tuning(narrow=1)
+    assert b.plan.get_tuning()["narrow"] == 1 and a.plan.get_tuning()["narrow"] == 0
+    sa, sb = _state(a.device), _state(b.device)
+    for k, n in enumerate((3, 1)):
+        print(f"[layout] narrow tiles N={N} {les} {precision} call {k}", flush=True)
+        _drive(a, fused.SCHEME_RK4, None, sa, n)
+        _drive(b, fused.SCHEME_RK4, None, sb, n)
+        torch.cuda.synchronize()
+        _same(a, b, k)
+        assert torch.equal(sa, sb)
