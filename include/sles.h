@@ -72,8 +72,9 @@ int sles_plan_info(sles_plan plan, int* n, int* nh, int* precision);
  *                             x-forward passes take tiles of N (N/2+1) / 1024
  *                             columns (four per CU; 1) instead of / 256 (one
  *                             per CU; 0, default) -- faster for one grid,
- *                             slower for batched launches, which fill the CUs
- *                             with their members; results are identical
+ *                             slower for grids that run side by side (batched
+ *                             launches, concurrent streams), which fill the
+ *                             CUs already; results are identical
  *                             (ignored at other sizes)
  * The "radix-4" alternative plans: 4 points per thread (radix 4) for
  * N = 32, 64, 128, 256, 512; 6 points per thread (radix 3, then 2) for

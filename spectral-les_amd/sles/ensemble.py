@@ -69,9 +69,6 @@ class Group:
         self.plan = d0.plan
         self.cfl_first = float(cfl_first)
         M = len(members)
-        if M > 1:
-            # the batch's grid rows already fill the CUs: one tile per CU
-            self.plan.set_tuning(narrow=0)
         dev = d0.device
         shape = (M,) + tuple(d0.U_hat.shape)
         self.U_hat = torch.empty(shape, dtype=d0.cdtype, device=dev)
@@ -133,6 +130,9 @@ class Ensemble:
                 Cs = float(rng.uniform(*cs_range))
                 d = InitializeDomain(N=N, nu=nu, method=method, LES_ON=1 if les_model else 0, LES_model=les_model,
                                      Cs=Cs, CFL=cfl, precision=precision)
+                # members run side by side (batched launches, or one stream
+                # each), which fills the CUs: one strided tile per CU
+                d.plan.set_tuning(narrow=0)
                 set_spectral_state(d, isotropic_spectrum(N, k0=k0, seed=seed * 1000 + i, urms=1.0, dtype=d.cdtype))
                 ds.append(d)
                 i += 1
