@@ -244,3 +244,27 @@ def test_narrow_tiles_drive_bitwise(N, les, precision):
         torch.cuda.synchronize()
         _same(a, b, k)
         assert torch.equal(sa, sb)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("pad", [0, 5])
+def test_drive_stopping_mid_call_equals_shorter_call(pad, precision):
+    """RK4 runs each step's last x-forward together with the next step's
+    first x-inverse inside a driver call.  A call whose final time stops it
+    after 2 of 4 steps must leave the state, U and curl exactly as a call of
+    those 2 steps does: the steps it skips hand on the x-inverse of the
+    unchanged U_hat, and the call's last step writes U and curl from it."""
+    from sles import fused
+
+    a = _domain(32, precision, "Smagorinsky", "RK4", pad)
+    b = _domain(32, precision, "Smagorinsky", "RK4", pad)
+    sa, sb = _state(a.device), _state(b.device)
+    print(f"[layout] stop mid-call pad={pad} {precision}", flush=True)
+    _drive(a, fused.SCHEME_RK4, None, sa, 2)
+    torch.cuda.synchronize()
+    sb[7] = sa[2].item()  # final time: reached by the second step
+    _drive(b, fused.SCHEME_RK4, None, sb, 4)
+    torch.cuda.synchronize()
+    _same(a, b, "stop")
+    assert sb[4].item() == 2 and sb[3].item() == 1.0  # steps taken, status DONE
+    assert sb[2].item() == sa[2].item()  # t
