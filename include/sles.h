@@ -55,7 +55,9 @@ int sles_plan_info(sles_plan plan, int* n, int* nh, int* precision);
  *                             step and the generic transforms)
  *   SLES_TUNE_FUSE_X          RK4 x-forward pass also runs the next stage's
  *                             x-inverse (1, default) or not (0), in both
- *                             precisions at every N
+ *                             precisions at every N; inside one
+ *                             sles_step_drive / sles_step_drive_batch call
+ *                             the last stage also runs the next step's
  *   SLES_TUNE_FFT_WIDE        generic transforms (sles_fft_r2c / sles_fft_c2r):
  *                             bit 0 = x pass, bit 1 = y pass run tiles of
  *                             twice the columns (256- or 512-byte row runs, LDS tile
