@@ -51,3 +51,14 @@ def test_python_binding_matches_header():
     from sles import _lib
 
     assert set(_lib._PROTOS) <= declared_symbols()
+
+
+def test_tuning_keys_match_header():
+    """sles/_lib.py's tuning-knob names map to the SLES_TUNE_* values the
+    header declares, one for one, and the autotuner only tunes known knobs."""
+    from sles import _lib, autotune
+
+    text = open(os.path.join(ROOT, "include", "sles.h")).read()
+    enum = {m.group(1).lower(): int(m.group(2)) for m in re.finditer(r"\bSLES_TUNE_([A-Z0-9_]+)\s*=\s*(\d+)", text)}
+    assert _lib.TUNE_KEYS == enum
+    assert set(autotune.KNOBS) <= set(_lib.TUNE_KEYS)
