@@ -7,9 +7,13 @@ device CFL reduction, the oracle's from compute_dt on its own U); every
 `every` steps it prints the velocity rel-L2 (FP64 sums over the (3, N, N, N)
 field), the relative dt difference and the largest relative difference of
 the calculate_Ek bins (src/SM_solver.py:154-192): against the peak bin over
-all bins, and relative over the bins above 1e-8 of the peak.
+all bins, and relative over the bins above 1e-8 of the peak.  With --driver
+the GPU side advances through sles_step_drive calls of `every` steps (the
+path of spectralMethodSolver and bench.py: padded copies of the state, the
+x-inverse fused across steps, U written at each call's end); otherwise one
+sles_step call per step.
 
-    python tests/parity_curves.py --n 64 --steps 500 --every 25
+    python tests/parity_curves.py --n 64 --steps 500 --every 25 [--driver]
 
 Test infrastructure (it runs the oracle): not collected by pytest; run on a
 machine with a GPU.
@@ -39,6 +43,7 @@ def main(argv=None):
     ap.add_argument("--every", type=int, default=25)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--les", default="Smagorinsky")
+    ap.add_argument("--driver", action="store_true", help="GPU steps through sles_step_drive calls of --every steps")
     a = ap.parse_args(argv)
 
     from sles import fused
@@ -59,22 +64,48 @@ def main(argv=None):
     o.U_hat1[:] = o.U_hat
     E = Ek_plot_variables(g, path=None)
     Eo = O.EkBins(o)
+    if a.driver:
+        import ctypes
+        import math
+
+        from sles import _lib
+
+        st = torch.zeros(8, dtype=torch.float64, device=g.device)
+        st[2] = math.nan  # t starts at the first dt
+        st[6] = math.inf  # no output pauses
+        st[7] = math.inf  # no final time
+        prm = fused.params(g, cfl=0.5)
+        w = fused.work(g)
+
+        def drive(n):
+            _lib.call("sles_step_drive", g.plan.handle, fused.SCHEME_RK4, ctypes.byref(prm), 0.3, int(n),
+                      g.U_hat.data_ptr(), g.U_hat0.data_ptr(), g.U_hat1.data_ptr(), None, g._U.data_ptr(),
+                      g.curl.data_ptr(), st.data_ptr(), w.data_ptr(), g.stream)
+    path = f"sles_step_drive, {a.every} steps per call" if a.driver else "one sles_step call per step"
     print(f"# N={N} {a.les} Cs=0.17 nu=1e-4 RK4 CFL 0.3 then 0.5, seed {a.seed}, FP64 GPU (fused step, device "
-          f"dt) vs oracle (numpy FP64, own dt)")
+          f"dt; {path}; tuning {g.plan.get_tuning()}) vs oracle (numpy FP64, own dt)")
     print("# E(k): calculate_Ek bins; |dE|/peak over all bins, and the largest relative difference over the bins")
     print("# holding >= 1e-8 of the peak bin (the others hold round-off of modes far below the largest)")
     print("# step   t             dt_gpu        |dt_g/dt_o-1|  vel_rel_L2  Ek_absdiff/peak  nres  Ek_rel_resolved")
     t = 0.0
     t0 = time.time()
     worst = 0.0
+    done = 0  # GPU steps taken (driver mode)
     for k in range(1, a.steps + 1):
         cfl = 0.3 if k == 1 else 0.5
-        fused.rk4_step(g, dt_device=True, cfl=cfl)
+        sample = k % a.every == 0 or k == a.steps
+        if not a.driver:
+            fused.rk4_step(g, dt_device=True, cfl=cfl)
+            dtg = float(g._scalars[1])
+        elif sample:
+            drive(k - done)
+            done = k
+            dtg = float(st[1])
+            assert int(st[4]) == k and int(st[3]) == 0, st.tolist()
         o.dt = O.compute_dt(o.U, O.find_max_velocity(o.U), o.nu, dx=o.dx, CFL=cfl)
         O.rk4_step(o)
-        dtg = float(g._scalars[1])
         t += o.dt
-        if k % a.every == 0 or k == a.steps:
+        if sample:
             Ug = g._U.cpu().numpy()
             Uo = np.array(o.U)
             r = float(np.linalg.norm((Ug - Uo).ravel()) / np.linalg.norm(Uo.ravel()))
