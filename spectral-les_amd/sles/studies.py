@@ -495,6 +495,9 @@ def main(argv=None):
     st.add_argument("--n", type=int, nargs="+", default=[256, 512])
     st.add_argument("--precision", nargs="+", default=["fp64", "fp32"])
     st.add_argument("--pad", type=int, nargs="+", default=[None], help="plane padding, 128-byte lines")
+    st.add_argument("--mixes", nargs="+", default=["9:9", "15:6", "12:6", "3:6", "1:1"],
+                    help="fields read:written per launch")
+    st.add_argument("--tiles", type=int, nargs="+", default=[0, 1, 2])
     ra = sub.add_parser("ensemble-adjoint")
     ra.add_argument("--steps", type=int, default=200)
     ra.add_argument("--out", default=None)
@@ -506,12 +509,15 @@ def main(argv=None):
                 json.dump(res, f, indent=1)
         return
     if a.cmd == "stream":
-        # mixes of the x passes: the x-forward + update of RK4 stages 0 / 1-2 / 3
-        # (9+9, 15+6, 12+6 fields), the x-inverse (3+6) and a plain 1+1
+        # default mixes of the x passes: the x-forward + update of RK4 stages
+        # 0 / 1-2 / 3 (9+9, 15+6, 12+6 fields), the x-inverse (3+6) and a
+        # plain 1+1; with the next x-inverse fused the x-forward writes 6 more
+        # (9:15 15:12 12:12)
+        mixes = tuple(tuple(int(v) for v in m.split(":")) for m in a.mixes)
         for n in a.n:
             for prec in a.precision:
                 for pad in a.pad:
-                    stream_ceiling(n, prec, mixes=((9, 9), (15, 6), (12, 6), (3, 6), (1, 1)), pad=pad)
+                    stream_ceiling(n, prec, mixes=mixes, tiles=tuple(a.tiles), pad=pad)
         return
     if a.cmd == "fft":
         fft_throughput(tuple(a.n), reps=a.reps)
